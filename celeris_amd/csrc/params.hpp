@@ -1,0 +1,94 @@
+// params.hpp -- launch parameter blocks shared by the host engine and the HIP kernels.
+//
+// Device memory layout (structure of arrays, FP32):
+//   row-major planes    : element (x, y) at [y * P + x],  P  = padded row pitch  (>= nx+4+16)
+//   transposed planes   : element (x, y) at [x * PT + y], PT = padded column pitch (>= ny+4+16)
+// x and y are ghost-inclusive indices, interior cells are [2, nx+2) x [2, ny+2).
+// hv and everything feeding the y-line solve live transposed so that both line
+// solves read contiguous memory.
+#pragma once
+
+namespace cel {
+
+constexpr int kBandRows = 16;   // rows one wave marches through in the explicit kernel
+constexpr int kStripCols = 60;  // interior columns one wave produces (64 lanes - 2 halo each side)
+
+struct ExplicitParams {
+    int nx, ny, P, PT;
+    int nstrips, nbands;
+    // state at step n
+    const float *w, *hu, *hvT;
+    // bottom: BY (north face), BX (east face), BA (centre)
+    const float *BY, *BX, *BA;
+    // Adams-Bashforth history: dU/dt at n-1 and n-2 (w, hu, hv planes), (F*, G*) at n-1, n-2
+    const float *Go[3], *Goo[3], *FGo[2], *FGoo[2];
+    // corrector inputs: predicted dU/dt and (F*, G*), and the state the predictor started from
+    const float *Gp[3], *FGp[2], *Cw, *Chu, *ChvT;
+    // outputs
+    float *w_new;    // w^{n+1} (interior)
+    float *Dx;       // right-hand side of the x-line systems (row-major)
+    float *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
+    float *Gs[3], *FGs[2];
+    // SimConstBuffer (compute.hlsl:65-94)
+    float two_theta, g, half_g, Bcoef_g, g_over_dx, g_over_dy, one_over_dx, one_over_dy;
+    float dt, epsilon, friction, sea_level;
+    int is_manning;
+    // variable-step AB3 coefficients (AdamsBashforthCoef) and derivative weights
+    float ab[3];
+    float der[9];
+    // level-0 tridiagonal coefficients: (BCOEF + 1/3), dx^2, dy^2 and their sea level
+    float kcoef, dx2, dy2, coef_sl;
+};
+
+struct LineParams {
+    int nx, ny, P, PT;
+    int nlines_x, nlines_y;         // blocks [0, nlines_x) solve x-lines, then y-lines
+    const float *Dx, *BA;           // x-lines: rhs row-major, bottom centre row-major
+    const float *DyT, *BAT;         // y-lines: rhs and bottom centre transposed
+    float *hu_out, *hvT_out;
+    float kcoef, dx2, dy2, coef_sl;
+    int Lx, Ly;                     // cyclic-reduction level counts
+    // irregular-wave strips, handled by blocks after the line solves
+    int n_irr_lines;                // total (side, strip-index, line) triples
+};
+
+enum : int { kSolid = 0, kSponge = 1, kSine = 2, kIrregular = 3, kUTS = 4 };
+enum : int { kWest = 0, kEast = 1, kSouth = 2, kNorth = 3 };
+
+struct IrregularSide {
+    int active;          // 1 if the side is IrregularWaves
+    int width;           // columns (W/E) or rows (S/N) of the strip
+    int nlines;          // cells along the strip (ny+4 for W/E, nx+4 for S/N)
+    int nleaf_active;    // leaves that can hold a component: min(n_waves, 2^(L-1))
+    int nleaf_pow2;      // smallest power of two >= nleaf_active (<= 2^(L-1))
+    int levels_above;    // log2(2^(L-1) / nleaf_pow2)
+    int zero_line;       // 1 if the reference copies an all-zero texture (L == 1)
+    float sl;
+    const float *amp, *omega, *phase;    // per component
+    const float *tab;                    // [(strip * nlines + line) * nw + wave] * 4: kx, ky, c*cos, c*sin
+    const unsigned char *wet;            // [strip * nlines + line]: d > MIN_DEPTH
+    int nw;                              // components stored per line
+    float *out;                          // [(strip * nlines + line) * 3]
+};
+
+struct SideParams {
+    int type, width;
+    float sl;
+    const float *tab;          // sponge: gamma; sine: kx, ky, c*cos, c*sin, wet (5 floats) per strip cell
+    float amp, omega;          // sine
+    float uts_eta, uts_hu, uts_hv;
+    const float *irr;          // irregular: strip values from IrregularSide::out
+};
+
+struct BoundaryParams {
+    int nx, ny, P, PT;
+    int X, Y;
+    int band_l, band_r, band_b, band_t;   // widths of the boundary cell set (>= 2 each)
+    int ncells;
+    int mirror_pass;                      // 1: cells whose chain passes a solid side; 0: all others
+    float t;
+    float *w, *hu, *hvT;
+    SideParams side[4];
+};
+
+}  // namespace cel

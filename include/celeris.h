@@ -1,0 +1,163 @@
+/*
+ * celeris.h -- C ABI of the MI355X-native Celeris Boussinesq time step.
+ *
+ * Drop-in for the numerical path of ShallowWaterEngine (engine.hpp:58-97,
+ * engine.cpp:849-1522): the step, its settings and its boundary-condition
+ * interface.  Plain pointers and sizes only; nothing throws across this ABI.
+ * Every call returns a cel_status; cel_last_error() describes the last failure
+ * on the calling thread.
+ *
+ * Array convention (host side): every plane is row-major, ghost-inclusive,
+ * (nx+4) columns by (ny+4) rows, x fastest -- the reference's texture layout
+ * (engine.cpp:3024-3034).  State planes are w, hu, hv; bottom planes are
+ * BY (north face), BX (east face), BA (centre), terrain_heightfield.cpp:254-256.
+ */
+#ifndef CELERIS_H
+#define CELERIS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Replace Coercri::DXError throws (engine.cpp:119-121, 345) and
+ * std::abort on unknown settings (settings.cpp:289). */
+typedef enum {
+    CEL_OK = 0,
+    CEL_EINVAL = 1,    /* bad argument or unsupported setting */
+    CEL_ENOMEM = 2,    /* device or host allocation failed */
+    CEL_EDEVICE = 3,   /* HIP runtime error */
+    CEL_ENUMERIC = 4,  /* non-finite state detected */
+    CEL_ESTATE = 5     /* call not valid in the engine's current state */
+} cel_status;
+
+/* BoundaryType, settings.hpp:90-96. */
+typedef enum {
+    CEL_SOLID = 0,
+    CEL_SPONGE = 1,
+    CEL_SINEWAVE = 2,
+    CEL_IRREGULARWAVES = 3,
+    CEL_UNIFORMTIMESERIES = 4
+} cel_boundary_type;
+
+/* BoundarySide, settings.hpp:98-103. */
+typedef enum { CEL_WEST = 0, CEL_EAST = 1, CEL_SOUTH = 2, CEL_NORTH = 3 } cel_side;
+
+/* TimeIntegrationScheme, settings.hpp:304. */
+typedef enum {
+    CEL_EULER = 0,
+    CEL_PREDICTOR = 1,
+    CEL_CORRECTOR = 2,
+    CEL_PREDICTOR_ADAPTIVE = 3
+} cel_time_scheme;
+
+/* Maximum irregular-wave components per side: MAX_NUM_OF_IRREGULAR_WAVES,
+ * engine.hpp:216 / compute.hlsl:111-114. */
+#define CEL_MAX_IRREGULAR_WAVES 1000
+/* Longest supported tridiagonal line (cells per side). */
+#define CEL_MAX_LINE 8192
+
+/* BoundarySetting, settings.hpp:149-159.  Tables are copied by the engine. */
+typedef struct {
+    int type;                 /* cel_boundary_type */
+    int width;                /* strip width in cells (reference default 2) */
+    float water_level;        /* waterLevel */
+    float amplitude;          /* SineWaveSetting.amplitude */
+    float period;             /* SineWaveSetting.period */
+    float theta;              /* SineWaveSetting.theta (radians from +x) */
+    int n_waves;              /* IrregularWaves: rows of (amplitude, period, theta, phase) */
+    const float *waves;
+    int n_uts;                /* UniformTimeSeries: rows of (t, eta, hu, hv) */
+    const float *uts;
+} cel_boundary;
+
+/* The InitSetting fields the step reads (settings.hpp:324-351), g from the
+ * "gravity" setting (gui_manager.cpp:107). */
+typedef struct {
+    int nx, ny;                   /* interior cells; arrays are (nx+4) x (ny+4) */
+    float width, length;          /* W (x extent) and L (y extent): dx = W/(nx-1), dy = L/(ny-1) */
+    float still_water_elevation;
+    int is_boussinesq;
+    float epsilon;
+    float theta;                  /* generalized-minmod theta (1..2) */
+    float friction;
+    int is_manning;
+    int correction_steps_num;
+    float timestep;               /* fixed dt, "uniform time step" */
+    int time_scheme;              /* CEL_PREDICTOR or CEL_PREDICTOR_ADAPTIVE */
+    float gravity;
+} cel_settings;
+
+/* Soliton, settings.hpp:180-192. */
+typedef struct {
+    float param_h, theta, length, xc, yc;
+} cel_soliton;
+
+typedef struct cel_engine cel_engine;
+
+/* Thread-local description of the last failed call (empty if none). */
+const char *cel_last_error(void);
+
+/* UpdateTerrainHeightfield, terrain_heightfield.cpp:244-298.  z holds the corner
+ * heights, (nx+5) x (ny+5), corner (i, j) at x = (i-2.5)/(nx-1)*W - W/2,
+ * y = (j-2.5)/(ny-1)*L.  Writes the three bottom planes BY, BX, BA. */
+int cel_bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
+
+/* Initial condition of createSimTextures, engine.cpp:2940-3016 (no hot-start
+ * file): w = max(SWL, BA) with clamped ghosts, plus solitary waves
+ * (engine.cpp:103-112, taper 2978-2999).  Writes the three state planes. */
+int cel_initial_state(const cel_settings *s, const float *bottom3, const cel_soliton *solitons,
+                      int n_solitons, float *state3);
+
+/* ShallowWaterEngine ctor + remesh(R_MESH) (engine.cpp:709-740): allocates the
+ * device state, uploads bottom and state, builds the per-cell boundary tables and
+ * resets the time scheme (Euler bootstrap, engine.cpp:3254-3269).
+ * bnd[4] is indexed by cel_side. */
+int cel_engine_create(const cel_settings *s, const cel_boundary bnd[4], const float *bottom3,
+                      const float *state3, cel_engine **out);
+int cel_engine_destroy(cel_engine *e);
+
+/* BoundarySetting change + createBoundaryShaders (engine.cpp:2405-2453). */
+int cel_engine_set_boundary(cel_engine *e, int side, const cel_boundary *b);
+
+/* resetTimestep with a caller-chosen dt (engine.cpp:1714-1729, 1750). */
+int cel_engine_set_dt(cel_engine *e, float dt);
+
+/* nsteps x { resetTimestep (fixed dt); timestep() } (main.cpp:396-425). Asynchronous
+ * on the engine's HIP stream. */
+int cel_engine_step(cel_engine *e, int nsteps);
+
+/* Blocks until queued steps finish. */
+int cel_engine_sync(cel_engine *e);
+
+/* Copies the current state (w, hu, hv planes) to the host (synchronises). */
+int cel_engine_get_state(cel_engine *e, float *state3);
+/* Hot start (engine.cpp:2943-2970): replaces the state; the AB history and the
+ * scheme counters are reset exactly as after remesh. */
+int cel_engine_set_state(cel_engine *e, const float *state3);
+
+/* Virtual time total_time (engine.cpp:1518) and the scheme of the next step. */
+int cel_engine_time(cel_engine *e, double *t);
+int cel_engine_scheme(cel_engine *e, int *scheme);
+
+/* Runs nsteps and returns the elapsed device time in ms, measured with HIP events
+ * on the engine's stream (inputs are already resident in HBM). */
+int cel_engine_timed_steps(cel_engine *e, int nsteps, float *ms);
+
+/* Per-kernel device time (ms) accumulated by cel_engine_timed_steps when profiling is
+ * on; names: "explicit", "solve_x", "solve_y", "boundary". */
+int cel_engine_set_profiling(cel_engine *e, int on);
+int cel_engine_kernel_time(cel_engine *e, const char *name, float *ms, int *launches);
+
+/* Device bytes held by the engine. */
+int cel_engine_device_bytes(cel_engine *e, size_t *bytes);
+
+/* Level-0 tridiagonal coefficients as the engine uses them (engine.cpp:2638-2694,
+ * 2781-2826): three planes A, B, C for x and for y, computed on the device. */
+int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,112 @@
+/*
+ * celeris_oracle.h -- CPU restatement of Celeris's Boussinesq time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle for the
+ * HIP implementation under celeris_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product never links it.
+ *
+ * It is a scalar transliteration of the pixel-shader passes in
+ * msvc/Release/shaders/compute.hlsl and of the pass sequencing in
+ * ShallowWaterEngine::timestep() (engine.cpp:849-1522), instantiated in two
+ * precisions: FP32 (same operation order as the HLSL, built with
+ * -ffp-contract=off; the parity judge) and FP64 (the accuracy judge).
+ * Every texture of the reference is kept as an RGBA array of the same size,
+ * so quirks of the reference (ghost zeroing, strip copy-back order, the
+ * gradient ring rotation) are reproduced literally.
+ *
+ * Array layout of every plane exchanged through this header: row-major,
+ * ghost-inclusive, (nx+4) columns by (ny+4) rows, x fastest.
+ */
+#ifndef CELERIS_ORACLE_H
+#define CELERIS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary types, settings.hpp:90-96 (BoundaryType). */
+enum { ORC_SOLID = 0, ORC_SPONGE = 1, ORC_SINEWAVE = 2, ORC_IRREGULAR = 3, ORC_UTS = 4 };
+/* Sides, settings.hpp (BoundarySide). */
+enum { ORC_WEST = 0, ORC_EAST = 1, ORC_SOUTH = 2, ORC_NORTH = 3 };
+/* Time schemes, settings.hpp:304 (TimeIntegrationScheme). */
+enum { ORC_EULER = 0, ORC_PREDICTOR = 1, ORC_CORRECTOR = 2, ORC_PREDICTOR_ADAPTIVE = 3 };
+
+/* BoundarySetting, settings.hpp:149-159. */
+typedef struct {
+    int type;
+    int width;
+    float water_level;
+    float amplitude, period, theta;   /* SineWaveSetting */
+    int n_waves;                      /* irregular table: (amp, period, theta, phase) x n_waves */
+    const float *waves;
+    int n_uts;                        /* uniform time series rows: (t, eta, hu, hv) x n_uts */
+    const float *uts;
+} orc_boundary;
+
+/* The InitSetting fields the step reads (settings.hpp:324-351). */
+typedef struct {
+    int nx, ny;
+    float width, length;              /* W (x), L (y); dx = W/(nx-1) */
+    float still_water_elevation;
+    int is_boussinesq;
+    float epsilon;
+    float theta;                      /* minmod theta; TWO_THETA = 2*theta */
+    float friction;
+    int is_manning;
+    int correction_steps_num;
+    float timestep;                   /* fixed dt ("uniform time step") */
+    int time_scheme;                  /* ORC_PREDICTOR or ORC_PREDICTOR_ADAPTIVE */
+    float gravity;
+    orc_boundary bnd[4];              /* indexed by ORC_WEST..ORC_NORTH */
+} orc_config;
+
+typedef struct orc_sim_f32 orc_sim_f32;
+typedef struct orc_sim_f64 orc_sim_f64;
+
+/* bottom: three planes BY, BX, BA (each (nx+4)*(ny+4)).
+ * state : three planes w, hu, hv.  Returns NULL on bad input. */
+orc_sim_f32 *orc_f32_create(const orc_config *cfg, const float *bottom, const float *state);
+void orc_f32_destroy(orc_sim_f32 *s);
+void orc_f32_step(orc_sim_f32 *s, int nsteps);
+/* Sets dt for the following steps (resetTimestep with an externally chosen dt). */
+void orc_f32_set_dt(orc_sim_f32 *s, float dt);
+void orc_f32_get_state(const orc_sim_f32 *s, float *state3);
+double orc_f32_time(const orc_sim_f32 *s);
+int orc_f32_scheme(const orc_sim_f32 *s);
+/* Level-0 tridiagonal coefficients (A, B, C planes) for x and y, engine.cpp:2638-2694, 2781-2826. */
+void orc_f32_coefficients(const orc_sim_f32 *s, float *abc_x3, float *abc_y3);
+
+orc_sim_f64 *orc_f64_create(const orc_config *cfg, const float *bottom, const float *state);
+void orc_f64_destroy(orc_sim_f64 *s);
+void orc_f64_step(orc_sim_f64 *s, int nsteps);
+void orc_f64_set_dt(orc_sim_f64 *s, float dt);
+void orc_f64_get_state(const orc_sim_f64 *s, double *state3);
+double orc_f64_time(const orc_sim_f64 *s);
+int orc_f64_scheme(const orc_sim_f64 *s);
+void orc_f64_coefficients(const orc_sim_f64 *s, double *abc_x3, double *abc_y3);
+
+/* Terrain preprocessing, terrain_heightfield.cpp:244-298: corner heights
+ * z[(ny+5)*(nx+5)] -> bottom planes BY, BX, BA. */
+void orc_bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
+
+/* Tridiagonal cyclic reduction exactly as the reference performs it
+ * (coefficient pyramid engine.cpp:2696-2749 + CyclicReduceDx/CyclicSubstituteXx,
+ * compute.hlsl:835-918) for ONE line of n interior unknowns; A[0] and C[n-1]
+ * are taken as given.  Used by tests to pin the solver against a direct solve. */
+void orc_f32_cr_solve(int n, const float *a, const float *b, const float *c, const float *d, float *x);
+void orc_f64_cr_solve(int n, const double *a, const double *b, const double *c, const double *d, double *x);
+
+/* Per-function entry points used by the known-answer tests (compute.hlsl helpers). */
+float orc_f32_minmod(float a, float b, float c);
+void orc_f32_reconstruct(float two_theta, float west, float here, float east, float *out_west, float *out_east);
+void orc_f32_correct_w(float B_west, float B_east, float w_bar, float *w_west, float *w_east);
+float orc_f32_divide_by_h(float h, float epsilon);
+float orc_f32_numerical_flux(float aplus, float aminus, float Fplus, float Fminus, float Udiff);
+float orc_f32_sponge(float L, float x);
+float orc_f32_wavenumber(float omega, float d, float g);
+void orc_f32_ab_coef(float dt, float dt1, float dt2, float *coef3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
