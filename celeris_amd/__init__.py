@@ -1,0 +1,232 @@
+"""celeris_amd -- MI355X-native Celeris Boussinesq time step.
+
+Python view of the C ABI in include/celeris.h (ctypes).  The numerical path runs in
+libceleris_amd.so (hand-written HIP kernels for gfx950); there is no Python or CPU
+fallback: importing the engine without the built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libceleris_amd.so")
+
+# BoundaryType (settings.hpp:90-96), BoundarySide, TimeIntegrationScheme (settings.hpp:304)
+SOLID, SPONGE, SINEWAVE, IRREGULARWAVES, UNIFORMTIMESERIES = range(5)
+WEST, EAST, SOUTH, NORTH = range(4)
+EULER, PREDICTOR, CORRECTOR, PREDICTOR_ADAPTIVE = range(4)
+STATUS = {0: "CEL_OK", 1: "CEL_EINVAL", 2: "CEL_ENOMEM", 3: "CEL_EDEVICE", 4: "CEL_ENUMERIC", 5: "CEL_ESTATE"}
+
+_F = ctypes.POINTER(ctypes.c_float)
+
+
+class CelBoundary(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("width", ctypes.c_int), ("water_level", ctypes.c_float),
+                ("amplitude", ctypes.c_float), ("period", ctypes.c_float), ("theta", ctypes.c_float),
+                ("n_waves", ctypes.c_int), ("waves", _F), ("n_uts", ctypes.c_int), ("uts", _F)]
+
+
+class CelSettings(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("width", ctypes.c_float), ("length", ctypes.c_float),
+                ("still_water_elevation", ctypes.c_float), ("is_boussinesq", ctypes.c_int),
+                ("epsilon", ctypes.c_float), ("theta", ctypes.c_float), ("friction", ctypes.c_float),
+                ("is_manning", ctypes.c_int), ("correction_steps_num", ctypes.c_int), ("timestep", ctypes.c_float),
+                ("time_scheme", ctypes.c_int), ("gravity", ctypes.c_float)]
+
+
+class CelSoliton(ctypes.Structure):
+    _fields_ = [("param_h", ctypes.c_float), ("theta", ctypes.c_float), ("length", ctypes.c_float),
+                ("xc", ctypes.c_float), ("yc", ctypes.c_float)]
+
+
+# Every symbol include/celeris.h declares, with its argument types.
+SIGNATURES = {
+    "cel_last_error": ([], ctypes.c_char_p),
+    "cel_bottom_from_corners": ([ctypes.c_int, ctypes.c_int, _F, _F], ctypes.c_int),
+    "cel_initial_state": ([ctypes.POINTER(CelSettings), _F, ctypes.POINTER(CelSoliton), ctypes.c_int, _F], ctypes.c_int),
+    "cel_engine_create": ([ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
+                           ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "cel_engine_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "cel_engine_set_boundary": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CelBoundary)], ctypes.c_int),
+    "cel_engine_set_dt": ([ctypes.c_void_p, ctypes.c_float], ctypes.c_int),
+    "cel_engine_step": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_engine_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "cel_engine_get_state": ([ctypes.c_void_p, _F], ctypes.c_int),
+    "cel_engine_set_state": ([ctypes.c_void_p, _F], ctypes.c_int),
+    "cel_engine_time": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "cel_engine_scheme": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_engine_timed_steps": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
+    "cel_engine_set_profiling": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_engine_kernel_time": ([ctypes.c_void_p, ctypes.c_char_p, _F, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_engine_device_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "cel_engine_coefficients": ([ctypes.c_void_p, _F, _F], ctypes.c_int),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libceleris_amd.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m celeris_amd.build` "
+                           "(the HIP library is the only implementation of the step)")
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+class CeleriError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _check(status):
+    if status != 0:
+        raise CeleriError(status, load_library().cel_last_error().decode())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_F)
+
+
+def settings(nx, ny, width, length, timestep, still_water_elevation=0.0, is_boussinesq=1, epsilon=5e-8, theta=1.3,
+             friction=0.0, is_manning=0, correction_steps_num=0, time_scheme=PREDICTOR, gravity=9.81):
+    """cel_settings with the reference's defaults (settings.hpp:376-435; g from gui_manager.cpp:107)."""
+    return CelSettings(nx, ny, width, length, still_water_elevation, is_boussinesq, epsilon, theta, friction,
+                       is_manning, correction_steps_num, timestep, time_scheme, gravity)
+
+
+class Boundary:
+    """BoundarySetting (settings.hpp:149-159)."""
+
+    def __init__(self, type=SOLID, width=2, water_level=0.0, amplitude=0.0, period=0.0, theta=0.0, waves=None,
+                 uts=None):
+        self.type, self.width, self.water_level = type, width, water_level
+        self.amplitude, self.period, self.theta = amplitude, period, theta
+        self.waves = None if waves is None else np.ascontiguousarray(waves, dtype=np.float32).reshape(-1, 4)
+        self.uts = None if uts is None else np.ascontiguousarray(uts, dtype=np.float32).reshape(-1, 4)
+
+    def to_c(self):
+        nw = 0 if self.waves is None else self.waves.shape[0]
+        nu = 0 if self.uts is None else self.uts.shape[0]
+        return CelBoundary(self.type, self.width, self.water_level, self.amplitude, self.period, self.theta, nw,
+                           None if nw == 0 else _fptr(self.waves), nu, None if nu == 0 else _fptr(self.uts))
+
+
+def bottom_from_corners(nx, ny, z):
+    """UpdateTerrainHeightfield (terrain_heightfield.cpp:244-298): corners -> (BY, BX, BA)."""
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    assert z.shape == (ny + 5, nx + 5)
+    out = np.zeros((3, ny + 4, nx + 4), np.float32)
+    _check(load_library().cel_bottom_from_corners(nx, ny, _fptr(z), _fptr(out)))
+    return out
+
+
+def initial_state(s, bottom, solitons=()):
+    """createSimTextures' initial condition (engine.cpp:2945-3016)."""
+    bottom = np.ascontiguousarray(bottom, dtype=np.float32)
+    out = np.zeros((3, s.ny + 4, s.nx + 4), np.float32)
+    arr = (CelSoliton * max(1, len(solitons)))(*[CelSoliton(*sol) for sol in solitons])
+    _check(load_library().cel_initial_state(ctypes.byref(s), _fptr(bottom), arr, len(solitons), _fptr(out)))
+    return out
+
+
+class ShallowWaterEngine:
+    """The step, settings and boundary interface of ShallowWaterEngine (engine.hpp:58-97) on one GPU."""
+
+    def __init__(self, s, boundaries, bottom, state):
+        lib = load_library()
+        self.s = s
+        self.nx, self.ny = s.nx, s.ny
+        self._bnd = list(boundaries)
+        cb = (CelBoundary * 4)(*[b.to_c() for b in self._bnd])
+        bottom = np.ascontiguousarray(bottom, dtype=np.float32)
+        state = np.ascontiguousarray(state, dtype=np.float32)
+        h = ctypes.c_void_p()
+        _check(lib.cel_engine_create(ctypes.byref(s), cb, _fptr(bottom), _fptr(state), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(load_library().cel_engine_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_boundary(self, side, b):
+        self._bnd[side] = b
+        c = b.to_c()
+        _check(load_library().cel_engine_set_boundary(self._h, side, ctypes.byref(c)))
+
+    def reset_timestep(self, dt):
+        _check(load_library().cel_engine_set_dt(self._h, dt))
+
+    def timestep(self, n=1):
+        _check(load_library().cel_engine_step(self._h, n))
+
+    def sync(self):
+        _check(load_library().cel_engine_sync(self._h))
+
+    def get_state(self):
+        out = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)
+        _check(load_library().cel_engine_get_state(self._h, _fptr(out)))
+        return out
+
+    def set_state(self, state):
+        state = np.ascontiguousarray(state, dtype=np.float32)
+        _check(load_library().cel_engine_set_state(self._h, _fptr(state)))
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        _check(load_library().cel_engine_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    @property
+    def scheme(self):
+        v = ctypes.c_int()
+        _check(load_library().cel_engine_scheme(self._h, ctypes.byref(v)))
+        return v.value
+
+    def timed_steps(self, n):
+        ms = ctypes.c_float()
+        _check(load_library().cel_engine_timed_steps(self._h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def set_profiling(self, on):
+        _check(load_library().cel_engine_set_profiling(self._h, int(bool(on))))
+
+    def kernel_time(self, name):
+        ms, n = ctypes.c_float(), ctypes.c_int()
+        _check(load_library().cel_engine_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        _check(load_library().cel_engine_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def coefficients(self):
+        ax = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)
+        ay = np.zeros_like(ax)
+        _check(load_library().cel_engine_coefficients(self._h, _fptr(ax), _fptr(ay)))
+        return ax, ay

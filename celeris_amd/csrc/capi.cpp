@@ -1,0 +1,163 @@
+// capi.cpp -- extern "C" entry points (include/celeris.h).  Exceptions never cross the ABI:
+// each call maps cel::Error / std::bad_alloc to a status and records the message.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+
+namespace cel {
+void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
+void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3);
+}
+
+struct cel_engine {
+    cel::ShallowWaterEngine *impl;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        g_last_error.clear();
+        f();
+        return CEL_OK;
+    } catch (const cel::Error &e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return CEL_ENOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return CEL_EDEVICE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return CEL_EDEVICE;
+    }
+}
+
+cel::ShallowWaterEngine &eng(cel_engine *e)
+{
+    if (!e || !e->impl) throw cel::Error(CEL_EINVAL, "null engine");
+    return *e->impl;
+}
+}  // namespace
+
+extern "C" {
+
+const char *cel_last_error(void) { return g_last_error.c_str(); }
+
+int cel_bottom_from_corners(int nx, int ny, const float *z, float *bottom3)
+{
+    return guarded([&] {
+        if (nx < 4 || ny < 4 || !z || !bottom3) throw cel::Error(CEL_EINVAL, "bad arguments");
+        cel::bottom_from_corners(nx, ny, z, bottom3);
+    });
+}
+
+int cel_initial_state(const cel_settings *s, const float *bottom3, const cel_soliton *solitons, int n_solitons,
+                      float *state3)
+{
+    return guarded([&] {
+        if (!s || !bottom3 || !state3 || n_solitons < 0 || (n_solitons > 0 && !solitons))
+            throw cel::Error(CEL_EINVAL, "bad arguments");
+        if (s->nx < 4 || s->ny < 4) throw cel::Error(CEL_EINVAL, "nx and ny must be at least 4");
+        cel::initial_state(*s, bottom3, solitons, n_solitons, state3);
+    });
+}
+
+int cel_engine_create(const cel_settings *s, const cel_boundary bnd[4], const float *bottom3, const float *state3,
+                      cel_engine **out)
+{
+    return guarded([&] {
+        if (!s || !out) throw cel::Error(CEL_EINVAL, "null settings or output");
+        *out = nullptr;
+        auto *e = new cel_engine{nullptr};
+        try {
+            e->impl = new cel::ShallowWaterEngine(*s, bnd, bottom3, state3);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int cel_engine_destroy(cel_engine *e)
+{
+    return guarded([&] {
+        if (!e) return;
+        delete e->impl;
+        delete e;
+    });
+}
+
+int cel_engine_set_boundary(cel_engine *e, int side, const cel_boundary *b)
+{
+    return guarded([&] {
+        if (!b) throw cel::Error(CEL_EINVAL, "null boundary");
+        eng(e).setBoundary(side, *b);
+    });
+}
+
+int cel_engine_set_dt(cel_engine *e, float dt) { return guarded([&] { eng(e).resetTimestep(dt); }); }
+int cel_engine_step(cel_engine *e, int nsteps) { return guarded([&] { eng(e).step(nsteps); }); }
+int cel_engine_sync(cel_engine *e) { return guarded([&] { eng(e).sync(); }); }
+int cel_engine_get_state(cel_engine *e, float *state3) { return guarded([&] { eng(e).getState(state3); }); }
+int cel_engine_set_state(cel_engine *e, const float *state3) { return guarded([&] { eng(e).setState(state3); }); }
+
+int cel_engine_time(cel_engine *e, double *t)
+{
+    return guarded([&] {
+        if (!t) throw cel::Error(CEL_EINVAL, "null output");
+        *t = eng(e).time();
+    });
+}
+
+int cel_engine_scheme(cel_engine *e, int *scheme)
+{
+    return guarded([&] {
+        if (!scheme) throw cel::Error(CEL_EINVAL, "null output");
+        *scheme = eng(e).scheme();
+    });
+}
+
+int cel_engine_timed_steps(cel_engine *e, int nsteps, float *ms)
+{
+    return guarded([&] {
+        if (!ms) throw cel::Error(CEL_EINVAL, "null output");
+        *ms = eng(e).timedSteps(nsteps);
+    });
+}
+
+int cel_engine_set_profiling(cel_engine *e, int on) { return guarded([&] { eng(e).setProfiling(on != 0); }); }
+
+int cel_engine_kernel_time(cel_engine *e, const char *name, float *ms, int *launches)
+{
+    return guarded([&] {
+        if (!name) throw cel::Error(CEL_EINVAL, "null name");
+        if (!eng(e).kernelTime(name, ms, launches)) throw cel::Error(CEL_EINVAL, std::string("unknown kernel ") + name);
+    });
+}
+
+int cel_engine_device_bytes(cel_engine *e, size_t *bytes)
+{
+    return guarded([&] {
+        if (!bytes) throw cel::Error(CEL_EINVAL, "null output");
+        *bytes = eng(e).deviceBytes();
+    });
+}
+
+int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3)
+{
+    return guarded([&] {
+        if (!abc_x3 || !abc_y3) throw cel::Error(CEL_EINVAL, "null output");
+        eng(e).coefficients(abc_x3, abc_y3);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,650 @@
+// engine.cpp -- host orchestration of the Celeris step on one MI355X.
+//
+// One call of timestep() issues four kernels on the engine's stream:
+//   explicit_step<scheme>  (Pass1+Pass2+Pass3 fused, engine.cpp:955-1048)
+//   solve_lines            (x-line and y-line tridiagonal solves + irregular-wave lines,
+//                           engine.cpp:1060-1233 and 1811-2151)
+//   boundary_apply x2      (four sides in the order E, N, W, S, engine.cpp:1235-1438)
+// then advances the scheme state machine exactly as engine.cpp:1476-1521.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace cel {
+
+namespace {
+
+constexpr float kMinDepth = 0.0001f;   // MIN_DEPTH, compute.hlsl:1016
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+float sponge_function(float PI, float L, float x)   // compute.hlsl:1011-1014
+{
+    return 0.5f + 0.5f * std::cos(PI * std::max(L - x, 0.f) / L);
+}
+
+float wavenumber(float omega, float d, float g)     // compute.hlsl:1080-1084
+{
+    return omega * omega / (g * std::sqrt(std::tanh(omega * omega * d / g)));
+}
+
+}  // namespace
+
+int count_of_matrices(int n)
+{
+    int c = 1;
+    while (n > 1) { n /= 2; ++c; }
+    return c;
+}
+
+void ShallowWaterEngine::check(hipError_t e, const char *what) const
+{
+    if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? CEL_ENOMEM : CEL_EDEVICE,
+                                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3,
+                                       const float *state3)
+    : s_(s)
+{
+    if (!bottom3 || !state3 || !bnd) throw Error(CEL_EINVAL, "null bottom, state or boundary array");
+    if (s.nx < 4 || s.ny < 4) throw Error(CEL_EINVAL, "nx and ny must be at least 4");
+    if (s.nx > CEL_MAX_LINE || s.ny > CEL_MAX_LINE) throw Error(CEL_EINVAL, "nx or ny above CEL_MAX_LINE");
+    if (!(s.width > 0.f) || !(s.length > 0.f)) throw Error(CEL_EINVAL, "width and length must be positive");
+    if (!(s.timestep > 0.f)) throw Error(CEL_EINVAL, "timestep must be positive");
+    if (!(s.gravity > 0.f)) throw Error(CEL_EINVAL, "gravity must be positive");
+    if (s.time_scheme != CEL_PREDICTOR && s.time_scheme != CEL_PREDICTOR_ADAPTIVE)
+        throw Error(CEL_EINVAL, "time_scheme must be CEL_PREDICTOR or CEL_PREDICTOR_ADAPTIVE");
+    if (s.correction_steps_num < 0) throw Error(CEL_EINVAL, "correction_steps_num must be >= 0");
+    nx_ = s.nx; ny_ = s.ny; X_ = nx_ + 4; Y_ = ny_ + 4;
+    P_ = (int)round_up((size_t)X_ + 16, 64);
+    PT_ = (int)round_up((size_t)Y_ + 16, 64);
+    Lx_ = count_of_matrices(nx_);
+    Ly_ = count_of_matrices(ny_);
+    dx_ = s.width / (float)(nx_ - 1);
+    dy_ = s.length / (float)(ny_ - 1);
+    PI_ = 4.0f * std::atan(1.0f);
+    for (int i = 0; i < 4; ++i) {
+        if (bnd[i].type < CEL_SOLID || bnd[i].type > CEL_UNIFORMTIMESERIES) throw Error(CEL_EINVAL, "bad boundary type");
+        side_[i].cfg = bnd[i];
+    }
+    check(hipGetDevice(&device_), "hipGetDevice");
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+    allocate();
+    uploadBottom(bottom3);
+    for (int i = 0; i < 4; ++i) setBoundary(i, bnd[i]);
+    setState(state3);
+}
+
+ShallowWaterEngine::~ShallowWaterEngine()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (int i = 0; i < 4; ++i) freeSideTables(i);
+    if (arena_) (void)hipFree(arena_);
+    for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void ShallowWaterEngine::allocate()
+{
+    const size_t plane = (size_t)P_ * Y_ * sizeof(float);
+    const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
+    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2;   // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu
+    const int ncol = 2 + 1 + 1 + 1;                       // hvT x2; BAT; DyT; ChvT
+    arena_bytes_ = plane * nrow + tplane * ncol;
+    void *mem = nullptr;
+    check(hipMalloc(&mem, arena_bytes_), "hipMalloc(state)");
+    arena_ = static_cast<char *>(mem);
+    check(hipMemsetAsync(arena_, 0, arena_bytes_, stream_), "hipMemset(state)");
+    char *q = arena_;
+    auto rp = [&]() { float *r = reinterpret_cast<float *>(q); q += plane; return r; };
+    auto cp = [&]() { float *r = reinterpret_cast<float *>(q); q += tplane; return r; };
+    w_[0] = rp(); w_[1] = rp(); hu_[0] = rp(); hu_[1] = rp();
+    BY_ = rp(); BX_ = rp(); BA_ = rp(); Dx_ = rp();
+    for (auto &g : G_) for (auto &c : g) c = rp();
+    for (auto &c : Gp_) c = rp();
+    for (auto &g : FG_) for (auto &c : g) c = rp();
+    for (auto &c : FGp_) c = rp();
+    Cw_ = rp(); Chu_ = rp();
+    hvT_[0] = cp(); hvT_[1] = cp(); BAT_ = cp(); DyT_ = cp(); ChvT_ = cp();
+}
+
+void ShallowWaterEngine::uploadBottom(const float *bottom3)
+{
+    const size_t N = (size_t)X_ * Y_;
+    float *dst[3] = {BY_, BX_, BA_};
+    for (int c = 0; c < 3; ++c)
+        check(hipMemcpy2DAsync(dst[c], (size_t)P_ * 4, bottom3 + c * N, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+                               hipMemcpyHostToDevice, stream_), "upload bottom");
+    hostBA_.assign(bottom3 + 2 * N, bottom3 + 3 * N);
+    std::vector<float> t((size_t)X_ * Y_);
+    for (int y = 0; y < Y_; ++y)
+        for (int x = 0; x < X_; ++x) t[(size_t)x * Y_ + y] = hostBA_[(size_t)y * X_ + x];
+    check(hipMemcpy2DAsync(BAT_, (size_t)PT_ * 4, t.data(), (size_t)Y_ * 4, (size_t)Y_ * 4, X_,
+                           hipMemcpyHostToDevice, stream_), "upload BA^T");
+    check(hipStreamSynchronize(stream_), "upload sync");
+}
+
+// createSimTextures ring/scheme init, engine.cpp:3254-3269, and resetTimestep's first
+// call (engine.cpp:1714-1720).
+void ShallowWaterEngine::resetScheme()
+{
+    cur_ = 0;
+    ring_old_ = 0; ring_oo_ = 1; ring_scr_ = 2;
+    time_scheme_now_ = CEL_EULER;
+    timestep_count_ = 1;
+    correction_steps_count_ = s_.correction_steps_num;
+    total_time_ = 0.f;
+    current_timestep_ = old_timestep_ = old_old_timestep_ = s_.timestep;
+    for (auto &sd : side_) sd.uts_iter = 1;
+}
+
+void ShallowWaterEngine::setState(const float *state3)
+{
+    if (!state3) throw Error(CEL_EINVAL, "null state");
+    check(hipStreamSynchronize(stream_), "sync");
+    const size_t N = (size_t)X_ * Y_;
+    for (int k = 0; k < 2; ++k) {
+        check(hipMemcpy2DAsync(w_[k], (size_t)P_ * 4, state3, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+                               hipMemcpyHostToDevice, stream_), "upload w");
+        check(hipMemcpy2DAsync(hu_[k], (size_t)P_ * 4, state3 + N, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+                               hipMemcpyHostToDevice, stream_), "upload hu");
+    }
+    std::vector<float> t(N);
+    const float *hv = state3 + 2 * N;
+    for (int y = 0; y < Y_; ++y)
+        for (int x = 0; x < X_; ++x) t[(size_t)x * Y_ + y] = hv[(size_t)y * X_ + x];
+    for (int k = 0; k < 2; ++k)
+        check(hipMemcpy2DAsync(hvT_[k], (size_t)PT_ * 4, t.data(), (size_t)Y_ * 4, (size_t)Y_ * 4, X_,
+                               hipMemcpyHostToDevice, stream_), "upload hv");
+    // history textures start at zero (engine.cpp:3024-3034 creates them without data)
+    const size_t plane = (size_t)P_ * Y_ * sizeof(float);
+    for (auto &g : G_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
+    for (auto &g : FG_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
+    check(hipStreamSynchronize(stream_), "upload sync");
+    resetScheme();
+}
+
+void ShallowWaterEngine::getState(float *state3)
+{
+    if (!state3) throw Error(CEL_EINVAL, "null state");
+    const size_t N = (size_t)X_ * Y_;
+    check(hipMemcpy2DAsync(state3, (size_t)X_ * 4, w_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
+                           hipMemcpyDeviceToHost, stream_), "download w");
+    check(hipMemcpy2DAsync(state3 + N, (size_t)X_ * 4, hu_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
+                           hipMemcpyDeviceToHost, stream_), "download hu");
+    std::vector<float> t(N);
+    check(hipMemcpy2DAsync(t.data(), (size_t)Y_ * 4, hvT_[cur_], (size_t)PT_ * 4, (size_t)Y_ * 4, X_,
+                           hipMemcpyDeviceToHost, stream_), "download hv");
+    check(hipStreamSynchronize(stream_), "download sync");
+    float *hv = state3 + 2 * N;
+    for (int y = 0; y < Y_; ++y)
+        for (int x = 0; x < X_; ++x) hv[(size_t)y * X_ + x] = t[(size_t)x * Y_ + y];
+}
+
+void ShallowWaterEngine::freeSideTables(int side)
+{
+    SideState &sd = side_[side];
+    for (void *p : {(void *)sd.d_tab, (void *)sd.d_irr_out, (void *)sd.d_irr_wave, (void *)sd.d_wet})
+        if (p) (void)hipFree(p);
+    sd.d_tab = nullptr; sd.d_irr_out = nullptr; sd.d_irr_wave = nullptr; sd.d_wet = nullptr;
+}
+
+void ShallowWaterEngine::setBoundary(int side, const cel_boundary &b)
+{
+    if (side < 0 || side > 3) throw Error(CEL_EINVAL, "side out of range");
+    if (b.type < CEL_SOLID || b.type > CEL_UNIFORMTIMESERIES) throw Error(CEL_EINVAL, "bad boundary type");
+    if (b.width < 0) throw Error(CEL_EINVAL, "negative boundary width");
+    if ((b.type == CEL_SINEWAVE) && !(b.period > 0.f)) throw Error(CEL_EINVAL, "sine-wave period must be positive");
+    if (b.type == CEL_IRREGULARWAVES) {
+        if (b.n_waves < 0 || b.n_waves > CEL_MAX_IRREGULAR_WAVES || (b.n_waves > 0 && !b.waves))
+            throw Error(CEL_EINVAL, "irregular wave table must hold 0..1000 rows");
+        for (int i = 0; i < b.n_waves; ++i)
+            if (!(b.waves[4 * i + 1] > 0.f)) throw Error(CEL_EINVAL, "irregular wave period must be positive");
+    }
+    if (b.type == CEL_UNIFORMTIMESERIES && (b.n_uts < 2 || !b.uts))
+        throw Error(CEL_EINVAL, "uniform time series needs at least two rows");
+    check(hipStreamSynchronize(stream_), "sync");
+    SideState &sd = side_[side];
+    const cel_boundary old = sd.cfg;
+    sd.cfg = b;
+    try {
+        validateBoundaries();
+    } catch (...) {
+        sd.cfg = old;
+        throw;
+    }
+    sd.waves.assign(b.waves ? b.waves : nullptr, b.waves ? b.waves + 4 * (size_t)std::max(b.n_waves, 0) : nullptr);
+    sd.uts.assign(b.uts ? b.uts : nullptr, b.uts ? b.uts + 4 * (size_t)std::max(b.n_uts, 0) : nullptr);
+    sd.cfg.waves = sd.waves.empty() ? nullptr : sd.waves.data();
+    sd.cfg.uts = sd.uts.empty() ? nullptr : sd.uts.data();
+    sd.uts_iter = 1;
+    buildSideTables(side);
+}
+
+// The boundary kernel evaluates each boundary cell's chain of side operations from the
+// pre-boundary state.  Cells whose chain passes a solid side run in the first launch; they
+// must not be the mirror source of another such cell.
+void ShallowWaterEngine::validateBoundaries() const
+{
+    int w[4];
+    for (int i = 0; i < 4; ++i) w[i] = side_[i].cfg.width;
+    const int bl = std::max(w[CEL_WEST], 2), br = std::max(w[CEL_EAST], 2);
+    const int bb = std::max(w[CEL_SOUTH], 2), bt = std::max(w[CEL_NORTH], 2);
+    if (bl + br > X_ || bb + bt > Y_) throw Error(CEL_EINVAL, "boundary strips overlap across the domain");
+    auto in_strip = [&](int sd, int x, int y) {
+        switch (sd) {
+        case CEL_WEST: return x < w[sd];
+        case CEL_EAST: return x >= X_ - w[sd];
+        case CEL_SOUTH: return y < w[sd];
+        default: return y >= Y_ - w[sd];
+        }
+    };
+    const int order[4] = {CEL_EAST, CEL_NORTH, CEL_WEST, CEL_SOUTH};
+    auto walk = [&](int x, int y, int &sx, int &sy) {
+        bool mirrored = false;
+        for (int k = 3; k >= 0; --k) {
+            const int sd = order[k];
+            if (!in_strip(sd, x, y)) continue;
+            const int ty = side_[sd].cfg.type;
+            if (ty == CEL_SOLID) {
+                mirrored = true;
+                if (sd == CEL_WEST) x = 4 - x;
+                else if (sd == CEL_SOUTH) y = 4 - y;
+                else if (sd == CEL_EAST) x = 2 * nx_ + 2 - x;
+                else y = 2 * ny_ + 2 - y;
+            } else if (ty != CEL_SPONGE) {
+                break;
+            }
+        }
+        sx = x; sy = y;
+        return mirrored;
+    };
+    auto boundary_cell = [&](int x, int y) { return x < bl || x >= X_ - br || y < bb || y >= Y_ - bt; };
+    for (int y = 0; y < Y_; ++y)
+        for (int x = 0; x < X_; ++x) {
+            if (!boundary_cell(x, y)) continue;
+            int sx, sy;
+            if (!walk(x, y, sx, sy)) continue;
+            if (sx == x && sy == y) continue;
+            if (sx < 0 || sy < 0 || sx >= X_ || sy >= Y_) continue;
+            if (!boundary_cell(sx, sy)) continue;
+            int tx, ty2;
+            if (walk(sx, sy, tx, ty2))
+                throw Error(CEL_EINVAL, "solid boundary strips this wide are not supported (mirror source is itself a mirrored cell)");
+        }
+}
+
+// Per-strip-cell tables: sponge damping factors (they depend on dt), sine-wave phase
+// terms, irregular-wave component terms.  Computed in FP32 with the reference formulas.
+void ShallowWaterEngine::buildSideTables(int side)
+{
+    freeSideTables(side);
+    SideState &sd = side_[side];
+    const cel_boundary &b = sd.cfg;
+    const int wdt = b.width;
+    const bool xs = side == CEL_WEST || side == CEL_EAST;
+    const int nlines = xs ? Y_ : X_;
+    const int ncell = wdt * nlines;
+    const float g = s_.gravity, sl = b.water_level;
+    // strip cell (strip index s, line position ln) -> grid cell
+    auto cell_xy = [&](int s, int ln, int &x, int &y) {
+        switch (side) {
+        case CEL_WEST: x = s; y = ln; break;
+        case CEL_EAST: x = X_ - wdt + s; y = ln; break;
+        case CEL_SOUTH: x = ln; y = s; break;
+        default: x = ln; y = Y_ - wdt + s; break;
+        }
+    };
+    auto BAat = [&](int x, int y) { return hostBA_[(size_t)y * X_ + x]; };
+    if (ncell == 0) return;
+    if (b.type == CEL_SPONGE) {
+        std::vector<float> gam(ncell);
+        const float del = xs ? dx_ : dy_;
+        const float Lw = (float)wdt * del;
+        for (int s = 0; s < wdt; ++s)
+            for (int ln = 0; ln < nlines; ++ln) {
+                int x, y;
+                cell_xy(s, ln, x, y);
+                const float d_here = std::max(kMinDepth, sl - BAat(x, y));
+                float dist;
+                switch (side) {
+                case CEL_WEST: dist = (float)x * del; break;
+                case CEL_SOUTH: dist = (float)y * del; break;
+                case CEL_EAST: dist = (float)(X_ - x) * del; break;
+                default: dist = (float)(Y_ - y) * del; break;
+                }
+                gam[(size_t)s * nlines + ln] = sponge_function(PI_, Lw, dist) /
+                    sponge_function(PI_, Lw, dist + std::sqrt(g * d_here) * current_timestep_);
+            }
+        check(hipMalloc(&sd.d_tab, gam.size() * 4), "hipMalloc(sponge)");
+        check(hipMemcpy(sd.d_tab, gam.data(), gam.size() * 4, hipMemcpyHostToDevice), "upload sponge");
+        table_bytes_ += gam.size() * 4;
+    } else if (b.type == CEL_SINEWAVE) {
+        std::vector<float> tab((size_t)ncell * 5, 0.f);
+        const float omega = 2 * PI_ / b.period;
+        sd.sine_omega = omega;
+        for (int s = 0; s < wdt; ++s)
+            for (int ln = 0; ln < nlines; ++ln) {
+                int x, y;
+                cell_xy(s, ln, x, y);
+                const float d_here = std::max(0.f, sl - BAat(x, y));
+                float px, py;
+                switch (side) {
+                case CEL_EAST: px = (float)(x - X_) * dx_; py = (float)y * dy_; break;
+                case CEL_NORTH: px = (float)x * dx_; py = (float)(y - Y_) * dy_; break;
+                default: px = (float)x * dx_; py = (float)y * dy_; break;
+                }
+                float *t = &tab[((size_t)s * nlines + ln) * 5];
+                if (d_here > kMinDepth) {
+                    const float k = wavenumber(omega, d_here, g);
+                    const float c = omega / k;
+                    t[0] = std::cos(b.theta) * px * k;
+                    t[1] = std::sin(b.theta) * py * k;
+                    t[2] = c * std::cos(b.theta);
+                    t[3] = c * std::sin(b.theta);
+                    t[4] = 1.f;
+                }
+            }
+        check(hipMalloc(&sd.d_tab, tab.size() * 4), "hipMalloc(sine)");
+        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice), "upload sine");
+        table_bytes_ += tab.size() * 4;
+    } else if (b.type == CEL_IRREGULARWAVES) {
+        const int L = xs ? Lx_ : Ly_;
+        const int pn = 1 << (L - 1);
+        const int nact = std::min(b.n_waves, pn);
+        sd.irr_nw = nact;
+        std::vector<float> wave((size_t)std::max(nact, 1) * 3, 0.f);
+        for (int i = 0; i < nact; ++i) {
+            wave[i] = b.waves[4 * i];
+            wave[std::max(nact, 1) + i] = 2 * PI_ / b.waves[4 * i + 1];
+            wave[2 * std::max(nact, 1) + i] = b.waves[4 * i + 3];
+        }
+        std::vector<float> tab((size_t)ncell * std::max(nact, 1) * 4, 0.f);
+        std::vector<unsigned char> wet(ncell, 0);
+        for (int s = 0; s < wdt; ++s)
+            for (int ln = 0; ln < nlines; ++ln) {
+                const size_t cell = (size_t)s * nlines + ln;
+                if (ln < 2 || ln >= nlines - 2) continue;
+                int col_or_row = xs ? (side == CEL_WEST ? s : X_ - wdt + s) : (side == CEL_SOUTH ? s : Y_ - wdt + s);
+                float px, py, Bh;
+                if (xs) {
+                    px = side == CEL_WEST ? (float)col_or_row * dx_ : (float)(col_or_row - X_) * dx_;
+                    py = (float)ln * dy_;
+                    Bh = BAat(col_or_row, ln);
+                } else {
+                    px = (float)ln * dx_;
+                    py = side == CEL_SOUTH ? (float)col_or_row * dy_ : (float)(col_or_row - Y_) * dy_;
+                    Bh = BAat(ln, col_or_row);
+                }
+                const float d_here = std::max(0.f, sl - Bh);
+                wet[cell] = d_here > kMinDepth;
+                if (!wet[cell]) continue;
+                for (int i = 0; i < nact; ++i) {
+                    const float theta = b.waves[4 * i + 2];
+                    const float omega = 2 * PI_ / b.waves[4 * i + 1];
+                    const float k = wavenumber(omega, d_here, g);
+                    const float c = omega / k;
+                    float *t = &tab[(cell * nact + i) * 4];
+                    t[0] = std::cos(theta) * px * k;
+                    t[1] = std::sin(theta) * py * k;
+                    t[2] = c * std::cos(theta);
+                    t[3] = c * std::sin(theta);
+                }
+            }
+        check(hipMalloc(&sd.d_tab, tab.size() * 4), "hipMalloc(irregular)");
+        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice), "upload irregular");
+        check(hipMalloc(&sd.d_irr_wave, wave.size() * 4), "hipMalloc(irregular waves)");
+        check(hipMemcpy(sd.d_irr_wave, wave.data(), wave.size() * 4, hipMemcpyHostToDevice), "upload waves");
+        check(hipMalloc(&sd.d_wet, wet.size()), "hipMalloc(wet)");
+        check(hipMemcpy(sd.d_wet, wet.data(), wet.size(), hipMemcpyHostToDevice), "upload wet");
+        check(hipMalloc(&sd.d_irr_out, (size_t)ncell * 3 * 4), "hipMalloc(irregular out)");
+        table_bytes_ += tab.size() * 4 + wave.size() * 4 + wet.size() + (size_t)ncell * 12;
+    }
+}
+
+void ShallowWaterEngine::resetTimestep(float dt)
+{
+    if (!(dt > 0.f)) throw Error(CEL_EINVAL, "dt must be positive");
+    if (dt != current_timestep_) {
+        check(hipStreamSynchronize(stream_), "sync");
+        current_timestep_ = dt;
+        for (int i = 0; i < 4; ++i)
+            if (side_[i].cfg.type == CEL_SPONGE) buildSideTables(i);
+    }
+}
+
+// timeSeriesHelper, engine.cpp:1893-1912
+void ShallowWaterEngine::timeSeries(int side, float t, float &eta, float &hu, float &hv)
+{
+    SideState &sd = side_[side];
+    const int n = (int)sd.uts.size() / 4;
+    int &it = sd.uts_iter;
+    eta = 0.f; hu = 0.f; hv = 0.f;
+    while (it < n && sd.uts[4 * it] < t) ++it;
+    if (it >= n) return;
+    const float *a = &sd.uts[4 * (it - 1)], *b = &sd.uts[4 * it];
+    const float m = (t - a[0]) / (b[0] - a[0]);
+    eta = a[1] + (b[1] - a[1]) * m;
+    hu = a[2] + (b[2] - a[2]) * m;
+    hv = a[3] + (b[3] - a[3]) * m;
+}
+
+void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
+{
+    std::memset(&p, 0, sizeof(p));
+    p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
+    p.nstrips = (nx_ + kStripCols - 1) / kStripCols;
+    p.nbands = (ny_ + kBandRows - 1) / kBandRows;
+    const int c = cur_, n = 1 - cur_;
+    p.w = w_[c]; p.hu = hu_[c]; p.hvT = hvT_[c];
+    p.BY = BY_; p.BX = BX_; p.BA = BA_;
+    for (int k = 0; k < 3; ++k) { p.Go[k] = G_[ring_old_][k]; p.Goo[k] = G_[ring_oo_][k]; p.Gs[k] = G_[ring_scr_][k]; p.Gp[k] = Gp_[k]; }
+    for (int k = 0; k < 2; ++k) { p.FGo[k] = FG_[ring_old_][k]; p.FGoo[k] = FG_[ring_oo_][k]; p.FGs[k] = FG_[ring_scr_][k]; p.FGp[k] = FGp_[k]; }
+    p.Cw = Cw_; p.Chu = Chu_; p.ChvT = ChvT_;
+    p.w_new = w_[n]; p.Dx = Dx_; p.DyT = DyT_;
+    // fillConstantBuffers, engine.cpp:4099-4136
+    const float g = s_.gravity;
+    p.two_theta = 2 * s_.theta;
+    p.g = g;
+    p.half_g = 0.5f * g;
+    p.Bcoef_g = (1 / 15.0f) * g;
+    p.g_over_dx = g / dx_;
+    p.g_over_dy = g / dy_;
+    p.one_over_dx = 1.0f / dx_;
+    p.one_over_dy = 1.0f / dy_;
+    p.dt = current_timestep_;
+    const float m = std::max(dx_, dy_);
+    p.epsilon = std::min(s_.epsilon, (m * m) * (m * m));   // CalcEpsilon, engine.cpp:290-299
+    p.friction = s_.friction;
+    p.sea_level = s_.still_water_elevation;
+    p.is_manning = s_.is_manning;
+    // AdamsBashforthCoef and SecondOrderDerivatives weights (compute.hlsl:560-605)
+    const float dt = current_timestep_, d1 = old_timestep_, d2 = old_old_timestep_;
+    p.ab[0] = (dt / d1) * ((2 * dt + 6 * d1 + 3 * d2) / (d1 + d2)) + 6;
+    p.ab[1] = -(dt / d1) * ((2 * dt + 3 * d1 + 3 * d2) / (d2));
+    p.ab[2] = (dt / d2) * ((2 * dt + 3 * d1) / (d1 + d2));
+    p.der[0] = (2 * d1 + d2) / (d1 * (d1 + d2));
+    p.der[1] = -(d1 + d2) / (d1 * d2);
+    p.der[2] = (d1) / (d2 * (d1 + d2));
+    p.der[3] = (d2) / (d1 * (d1 + d2));
+    p.der[4] = (d1 - d2) / (d1 * d2);
+    p.der[5] = -(d1) / (d2 * (d1 + d2));
+    p.der[6] = -(d2) / (d1 * (d1 + d2));
+    p.der[7] = (d1 + d2) / (d1 * d2);
+    p.der[8] = -(d1 + 2 * d2) / (d2 * (d1 + d2));
+    p.kcoef = 1 / 15.0f + 1 / 3.0f;
+    p.dx2 = dx_ * dx_;
+    p.dy2 = dy_ * dy_;
+    p.coef_sl = s_.is_boussinesq ? s_.still_water_elevation : -9999.f;
+}
+
+void ShallowWaterEngine::fillLines(LineParams &p) const
+{
+    std::memset(&p, 0, sizeof(p));
+    p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
+    p.nlines_x = ny_; p.nlines_y = nx_;
+    p.Dx = Dx_; p.BA = BA_; p.DyT = DyT_; p.BAT = BAT_;
+    const int n = 1 - cur_;
+    p.hu_out = hu_[n]; p.hvT_out = hvT_[n];
+    p.kcoef = 1 / 15.0f + 1 / 3.0f;
+    p.dx2 = dx_ * dx_; p.dy2 = dy_ * dy_;
+    p.coef_sl = s_.is_boussinesq ? s_.still_water_elevation : -9999.f;
+    p.Lx = Lx_; p.Ly = Ly_;
+}
+
+void ShallowWaterEngine::timestep()
+{
+    const int scheme = time_scheme_now_;
+    const int n = 1 - cur_;
+    ExplicitParams ep;
+    fillExplicit(ep);
+    if (profiling_) check(hipEventRecord(ev_[0], stream_), "event");
+    check(launch_explicit(scheme, ep, stream_), "explicit_step");
+    if (profiling_) check(hipEventRecord(ev_[1], stream_), "event");
+    // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
+    // steps nothing ever reads them
+    if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
+        const size_t plane = (size_t)P_ * Y_ * 4, tplane = (size_t)PT_ * X_ * 4;
+        for (int k = 0; k < 3; ++k) check(hipMemcpyAsync(Gp_[k], G_[ring_scr_][k], plane, hipMemcpyDeviceToDevice, stream_), "copy");
+        for (int k = 0; k < 2; ++k) check(hipMemcpyAsync(FGp_[k], FG_[ring_scr_][k], plane, hipMemcpyDeviceToDevice, stream_), "copy");
+        check(hipMemcpyAsync(Cw_, w_[cur_], plane, hipMemcpyDeviceToDevice, stream_), "copy");
+        check(hipMemcpyAsync(Chu_, hu_[cur_], plane, hipMemcpyDeviceToDevice, stream_), "copy");
+        check(hipMemcpyAsync(ChvT_, hvT_[cur_], tplane, hipMemcpyDeviceToDevice, stream_), "copy");
+    }
+    // line solves + irregular-wave lines
+    LineParams lp;
+    fillLines(lp);
+    IrregularAll irr;
+    std::memset(&irr, 0, sizeof(irr));
+    irr.t = total_time_;
+    for (int s = 0; s < 4; ++s) {
+        const SideState &sd = side_[s];
+        if (sd.cfg.type != CEL_IRREGULARWAVES || sd.cfg.width == 0) continue;
+        const bool xs = s == CEL_WEST || s == CEL_EAST;
+        IrregularSide &S = irr.side[s];
+        const int L = xs ? Lx_ : Ly_;
+        const int pn = 1 << (L - 1);
+        S.active = 1;
+        S.width = sd.cfg.width;
+        S.nlines = xs ? Y_ : X_;
+        S.nleaf_active = sd.irr_nw;
+        int p2 = 1;
+        while (p2 < sd.irr_nw) p2 <<= 1;
+        S.nleaf_pow2 = p2;
+        int above = 0;
+        for (int v = p2; v < pn; v <<= 1) ++above;
+        S.levels_above = above;
+        S.zero_line = L == 1;
+        S.sl = sd.cfg.water_level;
+        const int nw1 = std::max(sd.irr_nw, 1);
+        S.amp = sd.d_irr_wave; S.omega = sd.d_irr_wave + nw1; S.phase = sd.d_irr_wave + 2 * nw1;
+        S.tab = sd.d_tab; S.wet = sd.d_wet; S.nw = nw1 == 0 ? 1 : sd.irr_nw; S.out = sd.d_irr_out;
+        if (sd.irr_nw == 0) S.nw = 1;
+        irr.count[s] = S.width * S.nlines;
+    }
+    check(launch_solve(lp, irr, stream_), "solve_lines");
+    if (profiling_) check(hipEventRecord(ev_[2], stream_), "event");
+    // boundaries (engine.cpp:1235-1438), boundary time = t^n (engine.cpp:1324)
+    BoundaryParams bp;
+    std::memset(&bp, 0, sizeof(bp));
+    bp.nx = nx_; bp.ny = ny_; bp.P = P_; bp.PT = PT_; bp.X = X_; bp.Y = Y_;
+    bp.band_l = std::max(side_[CEL_WEST].cfg.width, 2);
+    bp.band_r = std::max(side_[CEL_EAST].cfg.width, 2);
+    bp.band_b = std::max(side_[CEL_SOUTH].cfg.width, 2);
+    bp.band_t = std::max(side_[CEL_NORTH].cfg.width, 2);
+    bp.ncells = (bp.band_l + bp.band_r) * Y_ + (X_ - bp.band_l - bp.band_r) * (bp.band_b + bp.band_t);
+    bp.t = total_time_;
+    bp.w = w_[n]; bp.hu = hu_[n]; bp.hvT = hvT_[n];
+    for (int s = 0; s < 4; ++s) {
+        SideParams &S = bp.side[s];
+        const SideState &sd = side_[s];
+        S.type = sd.cfg.type; S.width = sd.cfg.width; S.sl = sd.cfg.water_level;
+        S.tab = sd.d_tab; S.amp = sd.cfg.amplitude; S.omega = sd.sine_omega;
+        S.irr = sd.d_irr_out;
+        if (S.type == CEL_UNIFORMTIMESERIES) timeSeries(s, total_time_, S.uts_eta, S.uts_hu, S.uts_hv);
+    }
+    bp.mirror_pass = 1;
+    check(launch_boundary(bp, stream_), "boundary_apply");
+    bp.mirror_pass = 0;
+    check(launch_boundary(bp, stream_), "boundary_apply");
+    if (profiling_) {
+        check(hipEventRecord(ev_[3], stream_), "event");
+        check(hipEventSynchronize(ev_[3]), "event sync");
+        float ms;
+        check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event"); kt_[0].ms += ms; kt_[0].n++;
+        check(hipEventElapsedTime(&ms, ev_[1], ev_[2]), "event"); kt_[1].ms += ms; kt_[1].n++;
+        check(hipEventElapsedTime(&ms, ev_[2], ev_[3]), "event"); kt_[2].ms += ms; kt_[2].n++;
+    }
+    // time-scheme state machine for the next step (engine.cpp:1476-1521)
+    if (timestep_count_ >= 2) {
+        if (correction_steps_count_ == s_.correction_steps_num) {
+            time_scheme_now_ = s_.time_scheme;
+            correction_steps_count_ = 0;
+        } else {
+            time_scheme_now_ = CEL_CORRECTOR;
+            correction_steps_count_++;
+        }
+    }
+    if (time_scheme_now_ == CEL_PREDICTOR || time_scheme_now_ == CEL_PREDICTOR_ADAPTIVE || time_scheme_now_ == CEL_EULER) {
+        timestep_count_++;
+        const int t = ring_old_;
+        ring_old_ = ring_scr_;
+        ring_scr_ = ring_oo_;
+        ring_oo_ = t;
+        old_old_timestep_ = old_timestep_;
+        old_timestep_ = current_timestep_;
+        total_time_ += current_timestep_;
+    }
+    cur_ = n;
+}
+
+void ShallowWaterEngine::step(int nsteps)
+{
+    if (nsteps < 0) throw Error(CEL_EINVAL, "negative step count");
+    for (int i = 0; i < nsteps; ++i) timestep();
+}
+
+void ShallowWaterEngine::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+float ShallowWaterEngine::timedSteps(int nsteps)
+{
+    check(hipEventRecord(ev_[4], stream_), "event");
+    step(nsteps);
+    check(hipEventRecord(ev_[5], stream_), "event");
+    check(hipEventSynchronize(ev_[5]), "event sync");
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, ev_[4], ev_[5]), "event");
+    return ms;
+}
+
+bool ShallowWaterEngine::kernelTime(const std::string &name, float *ms, int *launches) const
+{
+    int i = -1;
+    if (name == "explicit_step") i = 0;
+    else if (name == "solve_lines") i = 1;
+    else if (name == "boundary_apply") i = 2;
+    if (i < 0) return false;
+    if (ms) *ms = kt_[i].ms;
+    if (launches) *launches = kt_[i].n;
+    return true;
+}
+
+void ShallowWaterEngine::coefficients(float *ax, float *ay)
+{
+    LineParams lp;
+    fillLines(lp);
+    const size_t N = (size_t)X_ * Y_;
+    float *d = nullptr;
+    check(hipMalloc(&d, N * 6 * 4), "hipMalloc");
+    check(launch_export_coefficients(lp, BA_, BAT_, d, d + 3 * N, stream_), "export_coefficients");
+    check(hipMemcpyAsync(ax, d, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
+    check(hipMemcpyAsync(ay, d + 3 * N, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
+    check(hipStreamSynchronize(stream_), "sync");
+    (void)hipFree(d);
+}
+
+}  // namespace cel

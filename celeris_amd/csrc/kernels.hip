@@ -1,0 +1,810 @@
+// kernels.hip -- gfx950 kernels of the Celeris Boussinesq step.
+//
+//   explicit_step   : MUSCL/minmod reconstruction + KP07 central-upwind fluxes + flux
+//                     divergence, bed slope, friction, Madsen-Sorensen dispersion and the
+//                     Euler / AB3 / AM4 / variable-step AB3 update, fused.  One wave
+//                     marches down a band of rows; the 64 lanes own 64 adjacent columns
+//                     (60 produce output, 2 halo lanes each side) and exchange
+//                     horizontal neighbours through cross-lane moves, vertical
+//                     neighbours stay in registers.  (compute.hlsl Pass1/Pass2/Pass3)
+//   solve_lines     : one workgroup per x-line or y-line; cyclic reduction in LDS with
+//                     the reference's reduced coefficients, recomputed from BA.
+//                     (engine.cpp:2612-2887 + compute.hlsl:835-947)
+//   irregular_lines : irregular-wave superposition with the reference's pairwise-tree
+//                     sum order (compute.hlsl:1196-1309), merged into the solve launch.
+//   boundary_apply  : the four boundary strips in the order E, N, W, S evaluated as one
+//                     chain per cell (compute.hlsl:985-1194, engine.cpp:1345-1438).
+//
+// Arithmetic keeps the shader's operation order; the build disables FMA contraction so
+// that every product and sum rounds as written.
+#include <hip/hip_runtime.h>
+#include "params.hpp"
+
+#define CEL_MAX_LINE_ 8192
+
+namespace cel {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b; }
+
+// lane i receives lane i+1 / i-1 (edge lanes get their own value; they are halo lanes)
+__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
+
+// MinMod, compute.hlsl:292-296
+__device__ __forceinline__ float minmod3(float a, float b, float c)
+{
+    return (a > 0.f && b > 0.f && c > 0.f) ? fminr(fminr(a, b), c)
+         : (a < 0.f && b < 0.f && c < 0.f) ? fmaxr(fmaxr(a, b), c) : 0.f;
+}
+
+// Reconstruct, compute.hlsl:298-315: half-slope of the generalized minmod limiter
+__device__ __forceinline__ float half_slope(float two_theta, float west, float here, float east)
+{
+    const float z1 = two_theta * (here - west);
+    const float z2 = (east - west);
+    const float z3 = two_theta * (east - here);
+    return 0.25f * minmod3(z1, z2, z3);
+}
+
+// CorrectW, compute.hlsl:317-328
+__device__ __forceinline__ void correct_w(float B_west, float B_east, float w_bar, float &w_west, float &w_east)
+{
+    if (w_east < B_east) {
+        w_east = B_east;
+        w_west = fmaxr(B_west, 2.f * w_bar - B_east);
+    } else if (w_west < B_west) {
+        w_east = fmaxr(B_east, 2.f * w_bar - B_west);
+        w_west = B_west;
+    }
+}
+
+// desingularised 1/h of CalcUV, compute.hlsl:330-340
+__device__ __forceinline__ float divide_by_h(float h, float eps)
+{
+    const float h2 = h * h;
+    const float h4 = h2 * h2;
+    return 1.41421356237309504880f * h / sqrtf(h4 + fmaxr(h4, eps));
+}
+
+// NumericalFlux, compute.hlsl:363-370
+__device__ __forceinline__ float num_flux(float ap, float am, float Fp, float Fm, float dU)
+{
+    if (ap - am != 0.f) return (ap * Fm - am * Fp + ap * am * dU) / (ap - am);
+    return 0.f;
+}
+
+struct Face { float h, u, v; };
+
+// Face values of one cell along one axis (Pass1, compute.hlsl:408-433): returns the
+// "low" face (W or S) and the "high" face (E or N).
+__device__ __forceinline__ void faces_1d(float two_theta, float eps,
+                                         float w_lo, float w_c, float w_hi,
+                                         float hu_lo, float hu_c, float hu_hi,
+                                         float hv_lo, float hv_c, float hv_hi,
+                                         float B_lo, float B_hi, Face &lo, Face &hi)
+{
+    const float sw = half_slope(two_theta, w_lo, w_c, w_hi);
+    float w_hi_f = w_c + sw, w_lo_f = w_c - sw;
+    const float su = half_slope(two_theta, hu_lo, hu_c, hu_hi);
+    const float hu_hi_f = hu_c + su, hu_lo_f = hu_c - su;
+    const float sv = half_slope(two_theta, hv_lo, hv_c, hv_hi);
+    const float hv_hi_f = hv_c + sv, hv_lo_f = hv_c - sv;
+    correct_w(B_lo, B_hi, w_c, w_lo_f, w_hi_f);
+    hi.h = w_hi_f - B_hi;
+    lo.h = w_lo_f - B_lo;
+    const float qh = divide_by_h(hi.h, eps);
+    hi.u = qh * hu_hi_f;
+    hi.v = qh * hv_hi_f;
+    const float ql = divide_by_h(lo.h, eps);
+    lo.u = ql * hu_lo_f;
+    lo.v = ql * hv_lo_f;
+}
+
+// Pass2 x-direction flux between cell E face (here) and the W face of the east cell.
+// normal velocity u, tangential v; for y-direction call with (v, u) swapped and the
+// momentum components mapped as in compute.hlsl:517-533.
+__device__ __forceinline__ void flux_x(float g, float half_g, const Face &E, const Face &Wn, float &f0, float &f1, float &f2)
+{
+    const float cE = sqrtf(g * E.h);
+    const float cW = sqrtf(g * Wn.h);
+    const float ap = fmaxr(fmaxr(E.u + cE, Wn.u + cW), 0.f);
+    const float am = fminr(fminr(E.u - cE, Wn.u - cW), 0.f);
+    f0 = num_flux(ap, am, Wn.h * Wn.u, E.h * E.u, Wn.h - E.h);
+    f1 = num_flux(ap, am, Wn.h * (Wn.u * Wn.u + half_g * Wn.h), E.h * (E.u * E.u + half_g * E.h),
+                  Wn.h * Wn.u - E.h * E.u);
+    f2 = num_flux(ap, am, Wn.h * Wn.u * Wn.v, E.h * E.u * E.v, Wn.h * Wn.v - E.h * E.v);
+}
+
+__device__ __forceinline__ void flux_y(float g, float half_g, const Face &N, const Face &Sn, float &f0, float &f1, float &f2)
+{
+    const float cN = sqrtf(g * N.h);
+    const float cS = sqrtf(g * Sn.h);
+    const float bp = fmaxr(fmaxr(N.v + cN, Sn.v + cS), 0.f);
+    const float bm = fminr(fminr(N.v - cN, Sn.v - cS), 0.f);
+    f0 = num_flux(bp, bm, Sn.h * Sn.v, N.h * N.v, Sn.h - N.h);
+    f1 = num_flux(bp, bm, Sn.h * Sn.u * Sn.v, N.h * N.u * N.v, Sn.h * Sn.u - N.h * N.u);
+    f2 = num_flux(bp, bm, Sn.h * (Sn.v * Sn.v + half_g * Sn.h), N.h * (N.v * N.v + half_g * N.h),
+                  Sn.h * Sn.v - N.h * N.v);
+}
+
+// Level-0 tridiagonal coefficients (engine.cpp:2664-2680): A, B, C of a line cell
+// from the depths at offsets -2..+2 along the line.
+__device__ __forceinline__ void line_coef(float kcoef, float d2, float dm2, float dm1, float d0, float dp1, float dp2,
+                                          bool first, bool last, float &A, float &B, float &C)
+{
+    const float K1 = kcoef * (d0 * d0 / d2);
+    const float K2 = (d0 / d2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
+    const float K2_72 = K2 / 72.f;
+    A = first ? 0.f : -K1 + K2_72;
+    B = 1.f + 2.f * K1;
+    C = last ? 0.f : -(K1 + K2_72);
+}
+
+// ------------------------------------------------------------------ explicit step
+
+// SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
+template <int SCHEME>
+__global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
+{
+    constexpr int R = kBandRows;
+    __shared__ float hvblk[R + 4][64];
+    __shared__ float dystg[R][64];
+    __shared__ float chvblk[SCHEME == 2 ? (R + 4) : 1][64];
+
+    const int lane = threadIdx.x;
+    const int wid = blockIdx.x;
+    const int strip = wid % p.nstrips;
+    const int band = wid / p.nstrips;
+    if (band >= p.nbands) return;
+    const int X = p.nx + 4;
+    const int x0 = 2 + strip * kStripCols;
+    const int x = x0 - 2 + lane;
+    const int xc = x < X ? x : X - 1;               // clamped column for loads
+    const bool out_lane = lane >= 2 && lane < 62 && x < p.nx + 2;
+    const int ys = 2 + band * R;
+    const int ye = min(ys + R, p.ny + 2);
+    const int P = p.P, PT = p.PT;
+
+    // transposed hv: each lane stages rows [ys-2, ye+2) of its own column (16-byte loads;
+    // ys-2 is a multiple of 4 and the column pitch leaves >= 16 floats of padding)
+    {
+        const int nrows = ye - ys + 4;
+        const float4 *src = reinterpret_cast<const float4 *>(p.hvT + (size_t)xc * PT + (ys - 2));
+        for (int r = 0; r < nrows; r += 4) {
+            const float4 v = src[r >> 2];
+            hvblk[r][lane] = v.x;
+            if (r + 1 < R + 4) hvblk[r + 1][lane] = v.y;
+            if (r + 2 < R + 4) hvblk[r + 2][lane] = v.z;
+            if (r + 3 < R + 4) hvblk[r + 3][lane] = v.w;
+        }
+        if (SCHEME == 2) {
+            const float4 *csrc = reinterpret_cast<const float4 *>(p.ChvT + (size_t)xc * PT + (ys - 2));
+            for (int r = 0; r < nrows; r += 4) {
+                const float4 v = csrc[r >> 2];
+                chvblk[r][lane] = v.x;
+                if (r + 1 < R + 4) chvblk[r + 1][lane] = v.y;
+                if (r + 2 < R + 4) chvblk[r + 2][lane] = v.z;
+                if (r + 3 < R + 4) chvblk[r + 3][lane] = v.w;
+            }
+        }
+    }
+    __syncthreads();
+
+    const float tt = p.two_theta, eps = p.epsilon, g = p.g, half_g = p.half_g;
+    const float sl = p.sea_level;
+
+    // register windows over rows; index 0 = y-2 ... 4 = y+2
+    float w_[5], BA_[5], hu_[4], hv_[4], BY_[4];
+    // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
+    auto ldrow = [&](const float *base, int row) { return base[(size_t)row * P + xc]; };
+
+    // prologue: rows ys-2 .. ys+1
+    for (int j = 0; j < 4; ++j) {
+        const int row = ys - 2 + j;
+        w_[j] = ldrow(p.w, row);
+        BA_[j] = ldrow(p.BA, row);
+    }
+    for (int j = 0; j < 3; ++j) {
+        const int row = ys - 1 + j;
+        hu_[j] = ldrow(p.hu, row);
+        hv_[j] = hvblk[row - (ys - 2)][lane];
+        BY_[j] = ldrow(p.BY, row);
+    }
+    const float hu_m2 = ldrow(p.hu, ys - 2);
+    const float hv_m2 = hvblk[0][lane];
+    const float BY_m2 = ldrow(p.BY, ys - 2);
+
+    // faces of row ys-1 (need rows ys-2..ys) -> keep N; faces of row ys -> S, N
+    Face Nprev, Scur, Ncur;
+    {
+        Face lo, hi;
+        faces_1d(tt, eps, w_[0], w_[1], w_[2], hu_m2, hu_[0], hu_[1], hv_m2, hv_[0], hv_[1], BY_m2, BY_[0], lo, hi);
+        Nprev = hi;
+        faces_1d(tt, eps, w_[1], w_[2], w_[3], hu_[0], hu_[1], hu_[2], hv_[0], hv_[1], hv_[2], BY_[0], BY_[1], lo, hi);
+        Scur = lo;
+        Ncur = hi;
+    }
+    float yf_s0, yf_s1, yf_s2;   // yflux at row y-1
+    flux_y(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
+
+    for (int y = ys; y < ye; ++y) {
+        // ---- load row y+2 (w, BA) and row y+2 for hu/hv/BY into slot 3
+        w_[4] = ldrow(p.w, y + 2);
+        BA_[4] = ldrow(p.BA, y + 2);
+        hu_[3] = ldrow(p.hu, y + 2);
+        hv_[3] = hvblk[y + 2 - (ys - 2)][lane];
+        BY_[3] = ldrow(p.BY, y + 2);
+        const float BXc = ldrow(p.BX, y);
+
+        // ---- faces of row y+1 along y: S face used now, N face kept
+        Face S1, N1;
+        faces_1d(tt, eps, w_[2], w_[3], w_[4], hu_[1], hu_[2], hu_[3], hv_[1], hv_[2], hv_[3], BY_[1], BY_[2], S1, N1);
+        float yf0, yf1, yf2;   // yflux at row y
+        flux_y(g, half_g, Ncur, S1, yf0, yf1, yf2);
+
+        // ---- faces of row y along x
+        const float wc = w_[2], huc = hu_[1], hvc = hv_[1];
+        const float wl = from_left(wc), wr = from_right(wc);
+        const float hul = from_left(huc), hur = from_right(huc);
+        const float hvl = from_left(hvc), hvr = from_right(hvc);
+        const float BXw = from_left(BXc);
+        Face Wf, Ef;
+        faces_1d(tt, eps, wl, wc, wr, hul, huc, hur, hvl, hvc, hvr, BXw, BXc, Wf, Ef);
+        // Pass1 stores u = (1/h)*hu and v = (1/h)*hv; along x the normal velocity is u.
+        Face Wn;
+        Wn.h = from_right(Wf.h);
+        Wn.u = from_right(Wf.u);
+        Wn.v = from_right(Wf.v);
+        float xf0, xf1, xf2;   // xflux at (x, y)
+        flux_x(g, half_g, Ef, Wn, xf0, xf1, xf2);
+        const float xw0 = from_left(xf0), xw1 = from_left(xf1), xw2 = from_left(xf2);
+
+        // ---- Pass3 at (x, y), compute.hlsl:616-812
+        const float wll = from_left(wl), wrr = from_right(wr);
+        const float wul = from_left(w_[3]), wur = from_right(w_[3]);
+        const float wdl = from_left(w_[1]), wdr = from_right(w_[1]);
+        const float BAc = BA_[2];
+        const float BAl = from_left(BAc), BAr = from_right(BAc);
+        const float BAll = from_left(BAl), BArr = from_right(BAr);
+        const float BYs = BY_[0];
+        const float BYc = BY_[1];
+
+        if (out_lane) {
+            const size_t i = (size_t)y * P + x;
+            const float h = fmaxr(0.f, wc - BAc);
+            const float d_here = fmaxr(0.f, sl - BAc);
+            const float d2_here = d_here * d_here;
+            const float d3_here = d2_here * d_here;
+            const float d_left = fmaxr(0.f, sl - BAl);
+            const float d_right = fmaxr(0.f, sl - BAr);
+            const float d_down = fmaxr(0.f, sl - BA_[1]);
+            const float d_up = fmaxr(0.f, sl - BA_[3]);
+            const float d_left_left = fmaxr(0.f, sl - BAll);
+            const float d_right_right = fmaxr(0.f, sl - BArr);
+            const float d_down_down = fmaxr(0.f, sl - BA_[0]);
+            const float d_up_up = fmaxr(0.f, sl - BA_[4]);
+
+            const float eta_here = wc - sl;
+            const float eta_left = wl - sl, eta_right = wr - sl;
+            const float eta_down = w_[1] - sl, eta_up = w_[3] - sl;
+            const float eta_left_left = wll - sl, eta_right_right = wrr - sl;
+            const float eta_down_down = w_[0] - sl, eta_up_up = w_[4] - sl;
+            const float eta_up_left = wul - sl, eta_up_right = wur - sl;
+            const float eta_down_left = wdl - sl, eta_down_right = wdr - sl;
+
+            const float odx = p.one_over_dx, ody = p.one_over_dy;
+            const float one_over_d2x = odx * odx;
+            const float one_over_d3x = one_over_d2x * odx;
+            const float one_over_d2y = ody * ody;
+            const float one_over_d3y = one_over_d2y * ody;
+
+            const float dd_by_dx = (-d_right_right + 8.f * d_right - 8.f * d_left + d_left_left) * odx / 12.f;
+            const float dd_by_dy = (-d_up_up + 8.f * d_up - 8.f * d_down + d_down_down) * ody / 12.f;
+            const float eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
+            const float eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
+            const float eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
+
+            const float Bg = p.Bcoef_g;
+            const float Psi1x = Bg * (d3_here) *
+                ((eta_right_right - 2.f * eta_right + 2.f * eta_left - eta_left_left) * (0.5f * one_over_d3x)
+                 + (eta_up_right - eta_up_left - 2.f * eta_right + 2.f * eta_left + eta_down_right - eta_down_left)
+                   * (0.5f * odx * one_over_d2y));
+            const float Psi2x = Bg * (d2_here) * (dd_by_dx * (2.f * eta_by_dx_dx + eta_by_dy_dy) + dd_by_dy * eta_by_dx_dy);
+            const float Psi1y = Bg * (d3_here) *
+                ((eta_up_up - 2.f * eta_up + 2.f * eta_down - eta_down_down) * (0.5f * one_over_d3y)
+                 + (eta_up_right + eta_up_left - 2.f * eta_up + 2.f * eta_down - eta_down_right - eta_down_left)
+                   * (0.5f * ody * one_over_d2x));
+            const float Psi2y = Bg * (d2_here) * (dd_by_dy * (eta_by_dx_dx + 2.f * eta_by_dy_dy) + dd_by_dx * eta_by_dx_dy);
+
+            // FrictionCalc, compute.hlsl:544-553 (pow(h,4) as (h*h)*(h*h))
+            float friction_ = 0.f;
+            if (p.friction != 0.f) {
+                float f = p.friction;
+                const float dbh = divide_by_h(h, eps);
+                if (p.is_manning == 1) f = g * p.friction * p.friction * powf(fabsf(dbh), 0.3333f);
+                friction_ = f * sqrtf(huc * huc + hvc * hvc) * dbh * dbh;
+            }
+            const float src_g = -p.g_over_dx * h * (BXc - BXw) - huc * friction_ + (Psi1x + Psi2x);
+            const float src_b = -p.g_over_dy * h * (BYc - BYs) - hvc * friction_ + (Psi1y + Psi2y);
+
+            const float dr = (xw0 - xf0) * odx + (yf_s0 - yf0) * ody + 0.f;
+            const float dg = (xw1 - xf1) * odx + (yf_s1 - yf1) * ody + src_g;
+            const float db = (xw2 - xf2) * odx + (yf_s2 - yf2) * ody + src_b;
+
+            // F*, G* (compute.hlsl:726-734); the "0 *" cross term of the reference adds zero
+            const float hv_up = hv_[2], hv_down = hv_[0];
+            const float hu_up = hu_[2], hu_down = hu_[0];
+            const float F_star = (1.f / 6.f * d_here *
+                (dd_by_dx * ((ody / 2.f) * (hv_up - hv_down)) + (dd_by_dy * ((odx / 2.f) * (hvr - hvl))))) + 0.f;
+            const float G_star = (1.f / 6.f * d_here *
+                (dd_by_dx * ((ody / 2.f) * (hu_up - hu_down)) + (dd_by_dy * ((odx / 2.f) * (hur - hul))))) + 0.f;
+
+            // level-0 coefficients of the x-line (row y) and y-line (column x) through this cell
+            float Ax, Bx, Cx, Ay, By, Cy;
+            {
+                const float cs = p.coef_sl;
+                const float e0 = fmaxr(0.f, cs - BAll), e1 = fmaxr(0.f, cs - BAl), e2 = fmaxr(0.f, cs - BAc);
+                const float e3 = fmaxr(0.f, cs - BAr), e4 = fmaxr(0.f, cs - BArr);
+                line_coef(p.kcoef, p.dx2, e0, e1, e2, e3, e4, x == 2, x == p.nx + 1, Ax, Bx, Cx);
+                const float f0 = fmaxr(0.f, cs - BA_[0]), f1 = fmaxr(0.f, cs - BA_[1]);
+                const float f3 = fmaxr(0.f, cs - BA_[3]), f4 = fmaxr(0.f, cs - BA_[4]);
+                line_coef(p.kcoef, p.dy2, f0, f1, e2, f3, f4, y == 2, y == p.ny + 1, Ay, By, Cy);
+            }
+
+            const float dt = p.dt;
+            float wOut, huOut, hvOut;
+            if (SCHEME == 0) {
+                wOut = wc + dt * dr;
+                huOut = Ax * hul + Bx * huc + Cx * hur + dt * dg;
+                hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt * db;
+            } else {
+                const float o0 = p.Go[0][i], o1 = p.Go[1][i], o2 = p.Go[2][i];
+                const float oo0 = p.Goo[0][i], oo1 = p.Goo[1][i], oo2 = p.Goo[2][i];
+                const float Fo = p.FGo[0][i], Go_ = p.FGo[1][i];
+                const float Foo = p.FGoo[0][i], Goo_ = p.FGoo[1][i];
+                if (SCHEME == 1) {
+                    wOut = wc + dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
+                    huOut = Ax * hul + Bx * huc + Cx * hur + dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1) +
+                            (2.f * F_star - 3.f * Fo + Foo);
+                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt / 12.f * (23.f * db - 16.f * o2 + 5.f * oo2) +
+                            (2.f * G_star - 3.f * Go_ + Goo_);
+                } else if (SCHEME == 3) {
+                    const float *c = p.der;
+                    const float dF0 = (c[0] * F_star) + (c[1] * Fo) + (c[2] * Foo);
+                    const float dF1 = (c[3] * F_star) + (c[4] * Fo) + (c[5] * Foo);
+                    const float dF2 = (c[6] * F_star) + (c[7] * Fo) + (c[8] * Foo);
+                    const float dG0 = (c[0] * G_star) + (c[1] * Go_) + (c[2] * Goo_);
+                    const float dG1 = (c[3] * G_star) + (c[4] * Go_) + (c[5] * Goo_);
+                    const float dG2 = (c[6] * G_star) + (c[7] * Go_) + (c[8] * Goo_);
+                    const float a0 = p.ab[0], a1 = p.ab[1], a2 = p.ab[2];
+                    wOut = wc + dt / 6.f * (a0 * dr + a1 * o0 + a2 * oo0);
+                    huOut = Ax * hul + Bx * huc + Cx * hur + dt / 6.f * (a0 * dg + a1 * o1 + a2 * oo1) +
+                            dt / 6.f * (a0 * dF0 + a1 * dF1 + a2 * dF2);
+                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2) +
+                            dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
+                } else {
+                    const float cw = p.Cw[i];
+                    const float chu = p.Chu[i], chul = p.Chu[i - 1], chur = p.Chu[i + 1];
+                    const float chv = chvblk[y - (ys - 2)][lane];
+                    const float chvd = chvblk[y - 1 - (ys - 2)][lane];
+                    const float chvu = chvblk[y + 1 - (ys - 2)][lane];
+                    const float pr0 = p.Gp[0][i], pr1 = p.Gp[1][i], pr2 = p.Gp[2][i];
+                    const float Fp = p.FGp[0][i], Gp_ = p.FGp[1][i];
+                    wOut = cw + dt / 24.f * (9.f * dr + 19.f * pr0 - 5.f * o0 + 1.f * oo0);
+                    huOut = Ax * chul + Bx * chu + Cx * chur + dt / 24.f * (9.f * dg + 19.f * pr1 - 5.f * o1 + 1.f * oo1) +
+                            F_star - Fp;
+                    hvOut = Ay * chvd + By * chv + Cy * chvu + dt / 24.f * (9.f * db + 19.f * pr2 - 5.f * o2 + 1.f * oo2) +
+                            G_star - Gp_;
+                }
+            }
+            p.w_new[i] = wOut;
+            p.Dx[i] = huOut;
+            dystg[y - ys][lane] = hvOut;
+            p.Gs[0][i] = dr;
+            p.Gs[1][i] = dg;
+            p.Gs[2][i] = db;
+            p.FGs[0][i] = F_star;
+            p.FGs[1][i] = G_star;
+        }
+        // ---- shift windows
+        w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];
+        BA_[0] = BA_[1]; BA_[1] = BA_[2]; BA_[2] = BA_[3]; BA_[3] = BA_[4];
+        hu_[0] = hu_[1]; hu_[1] = hu_[2]; hu_[2] = hu_[3];
+        hv_[0] = hv_[1]; hv_[1] = hv_[2]; hv_[2] = hv_[3];
+        BY_[0] = BY_[1]; BY_[1] = BY_[2]; BY_[2] = BY_[3];
+        Ncur = N1;
+        yf_s0 = yf0; yf_s1 = yf1; yf_s2 = yf2;
+    }
+    __syncthreads();
+    // transposed store of the y-line right-hand sides: column x, rows [ys, ye) at offset y-2
+    if (out_lane) {
+        float *dst = p.DyT + (size_t)x * PT + (ys - 2);
+        const int nr = ye - ys;
+        if (nr == R) {
+            float4 *d4 = reinterpret_cast<float4 *>(dst);
+            for (int r = 0; r < R; r += 4)
+                d4[r >> 2] = make_float4(dystg[r][lane], dystg[r + 1][lane], dystg[r + 2][lane], dystg[r + 3][lane]);
+        } else {
+            for (int r = 0; r < nr; ++r) dst[r] = dystg[r][lane];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ line solves
+
+// In-place cyclic reduction of one line held in LDS (a, b, c, d of n unknowns).
+// Level k keeps unknowns at positions 2^k (m+1) - 1; their neighbours at level k-1
+// sit 2^(k-1) away; a neighbour at or beyond n is the reference's (0, 1, 0 | 0) margin.
+// Back-substitution leaves the solution in d.  Arithmetic as in engine.cpp:2719-2735
+// and compute.hlsl:849-852, 913.
+template <int NT>
+__device__ void cr_solve_lds(int n, int L, float *a, float *b, float *c, float *d)
+{
+    const int tid = threadIdx.x;
+    for (int k = 1; k < L; ++k) {
+        const int s = 1 << (k - 1);
+        const int cnt = n >> k;
+        for (int m = tid; m < cnt; m += NT) {
+            const int pp = ((m + 1) << k) - 1;
+            const int pl = pp - s, pr = pp + s;
+            const float aL = a[pl], bL = b[pl], cL = c[pl], dL = d[pl];
+            float aR = 0.f, bR = 1.f, cR = 0.f, dR = 0.f;
+            if (pr < n) { aR = a[pr]; bR = b[pr]; cR = c[pr]; dR = d[pr]; }
+            const float ap = a[pp], bp = b[pp], cp = c[pp], dp = d[pp];
+            const float k1 = ap / bL;
+            const float k2 = cp / bR;
+            a[pp] = -aL * k1;
+            b[pp] = bp - cL * k1 - aR * k2;
+            c[pp] = -cR * k2;
+            d[pp] = dp - dL * k1 - dR * k2;
+        }
+        __syncthreads();
+    }
+    for (int k = L - 1; k >= 0; --k) {
+        const int s = 1 << k;
+        const int nk = n >> k;
+        const int cnt = (nk + 1) >> 1;
+        for (int q = tid; q < cnt; q += NT) {
+            const int pp = s * (2 * q + 1) - 1;
+            const float xl = (pp - s >= 0) ? d[pp - s] : 0.f;
+            const float xr = (pp + s < n) ? d[pp + s] : 0.f;
+            d[pp] = (d[pp] - a[pp] * xl - c[pp] * xr) / b[pp];
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kSolveThreads = 256;
+
+// Irregular waves: one wave per (side, strip index, line cell) computes that cell's
+// pairwise-tree sum of the component texels (ColumnSumReduce/RowSumReduce order).
+__device__ void irregular_line(const IrregularSide &S, int strip, int line, float t)
+{
+    const int lane = threadIdx.x & 63;
+    const int cell = strip * S.nlines + line;
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    const bool ghost = line < 2 || line >= S.nlines - 2 || S.zero_line;
+    if (!ghost) {
+        const bool wet = S.wet[cell] != 0;
+        const int A = S.nleaf_pow2;                 // power of two, <= 1024
+        const int lanes_used = A > 64 ? 64 : A;
+        const int nblk = A > 64 ? A / 64 : 1;       // blocks of 64 leaves
+        float b0 = 0.f, b1 = 0.f, b2 = 0.f;         // lane j keeps the sum of block j
+        for (int blk = 0; blk < nblk; ++blk) {
+            const int wv = blk * 64 + lane;
+            float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+            if (lane < lanes_used) {
+                float e = 0.f, hu = 0.f, hv = 0.f;
+                if (wet && wv < S.nleaf_active) {
+                    const float *tb = S.tab + ((size_t)cell * S.nw + wv) * 4;
+                    const float arg = S.omega[wv] * t - tb[0] - tb[1] + S.phase[wv];
+                    e = S.amp[wv] * sinf(arg);
+                    hu = tb[2] * e;
+                    hv = tb[3] * e;
+                }
+                v0 = e + S.sl; v1 = hu; v2 = hv;
+            }
+            // pairwise tree over the block's leaves (ColumnSumReduce order)
+            for (int off = 1; off < lanes_used; off <<= 1) {
+                const float o0 = __shfl_down(v0, off, 64), o1 = __shfl_down(v1, off, 64), o2 = __shfl_down(v2, off, 64);
+                if ((lane & (2 * off - 1)) == 0) { v0 = v0 + o0; v1 = v1 + o1; v2 = v2 + o2; }
+            }
+            v0 = __shfl(v0, 0, 64); v1 = __shfl(v1, 0, 64); v2 = __shfl(v2, 0, 64);
+            if (lane == blk) { b0 = v0; b1 = v1; b2 = v2; }
+        }
+        // upper levels of the tree over the block sums
+        for (int off = 1; off < nblk; off <<= 1) {
+            const float o0 = __shfl_down(b0, off, 64), o1 = __shfl_down(b1, off, 64), o2 = __shfl_down(b2, off, 64);
+            if ((lane & (2 * off - 1)) == 0) { b0 = b0 + o0; b1 = b1 + o1; b2 = b2 + o2; }
+        }
+        // subtrees beyond the active block hold only inactive texels (sl, 0, 0)
+        float c0 = S.sl, c1 = 0.f, c2 = 0.f;
+        for (int w = 1; w < A; w <<= 1) { c0 = c0 + c0; c1 = c1 + c1; c2 = c2 + c2; }
+        for (int l = 0; l < S.levels_above; ++l) {
+            b0 = b0 + c0; b1 = b1 + c1; b2 = b2 + c2;
+            c0 = c0 + c0; c1 = c1 + c1; c2 = c2 + c2;
+        }
+        r0 = b0; r1 = b1; r2 = b2;
+    }
+    if (lane == 0) {
+        S.out[(size_t)cell * 3 + 0] = r0;
+        S.out[(size_t)cell * 3 + 1] = r1;
+        S.out[(size_t)cell * 3 + 2] = r2;
+    }
+}
+
+
+__global__ __launch_bounds__(kSolveThreads) void solve_lines(const LineParams p, const IrregularAll irr)
+{
+    extern __shared__ float lds[];
+    const int bid = blockIdx.x;
+    const int nlines = p.nlines_x + p.nlines_y;
+    if (bid >= nlines) {
+        // irregular-wave lines: 4 per block (one per wave)
+        int idx = (bid - nlines) * (kSolveThreads / 64) + (threadIdx.x >> 6);
+        for (int s = 0; s < 4; ++s) {
+            if (idx < irr.count[s]) {
+                const IrregularSide &S = irr.side[s];
+                irregular_line(S, idx / S.nlines, idx % S.nlines, irr.t);
+                return;
+            }
+            idx -= irr.count[s];
+        }
+        return;
+    }
+    const bool xdir = bid < p.nlines_x;
+    const int n = xdir ? p.nx : p.ny;
+    const int L = xdir ? p.Lx : p.Ly;
+    const float del2 = xdir ? p.dx2 : p.dy2;
+    const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;   // row y or column x
+    const float *D = xdir ? p.Dx + (size_t)line * p.P + 2 : p.DyT + (size_t)line * p.PT;
+    const float *BAl = xdir ? p.BA + (size_t)line * p.P : p.BAT + (size_t)line * p.PT;
+    float *a = lds, *b = lds + n, *c = lds + 2 * n, *d = lds + 3 * n;
+    const float cs = p.coef_sl;
+    for (int i = threadIdx.x; i < n; i += kSolveThreads) {
+        const float e0 = fmaxr(0.f, cs - BAl[i]), e1 = fmaxr(0.f, cs - BAl[i + 1]), e2 = fmaxr(0.f, cs - BAl[i + 2]);
+        const float e3 = fmaxr(0.f, cs - BAl[i + 3]), e4 = fmaxr(0.f, cs - BAl[i + 4]);
+        float A, B, C;
+        line_coef(p.kcoef, del2, e0, e1, e2, e3, e4, i == 0, i == n - 1, A, B, C);
+        a[i] = A; b[i] = B; c[i] = C; d[i] = D[i];
+    }
+    __syncthreads();
+    cr_solve_lds<kSolveThreads>(n, L, a, b, c, d);
+    float *out = xdir ? p.hu_out + (size_t)line * p.P + 2 : p.hvT_out + (size_t)line * p.PT + 2;
+    for (int i = threadIdx.x; i < n; i += kSolveThreads) out[i] = d[i];
+}
+
+// ------------------------------------------------------------------ boundaries
+
+struct Val { float w, hu, hv; };
+
+__device__ __forceinline__ bool in_strip(const BoundaryParams &p, int side, int x, int y)
+{
+    const int wdt = p.side[side].width;
+    switch (side) {
+    case kWest: return x < wdt;
+    case kEast: return x >= p.X - wdt;
+    case kSouth: return y < wdt;
+    default: return y >= p.Y - wdt;
+    }
+}
+
+// position of a cell inside a side's strip table: (strip index) * line length + line position
+__device__ __forceinline__ int strip_index(const BoundaryParams &p, int side, int x, int y)
+{
+    switch (side) {
+    case kWest: return x * p.Y + y;
+    case kEast: return (x - (p.X - p.side[kEast].width)) * p.Y + y;
+    case kSouth: return y * p.X + x;
+    default: return (y - (p.Y - p.side[kNorth].width)) * p.X + x;
+    }
+}
+
+__device__ __forceinline__ Val pre_value(const BoundaryParams &p, int x, int y)
+{
+    Val v{0.f, 0.f, 0.f};
+    if (x >= 2 && x < p.nx + 2 && y >= 2 && y < p.ny + 2) {
+        v.w = p.w[(size_t)y * p.P + x];
+        v.hu = p.hu[(size_t)y * p.P + x];
+        v.hv = p.hvT[(size_t)x * p.PT + y];
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= p.ncells) return;
+    // enumerate the boundary cell set: left band, right band, then bottom/top bands between them
+    int x, y;
+    {
+        int r = tid;
+        const int nl = p.band_l * p.Y, nr = p.band_r * p.Y;
+        const int mid = p.X - p.band_l - p.band_r;
+        if (r < nl) { x = r / p.Y; y = r % p.Y; }
+        else if ((r -= nl) < nr) { x = p.X - p.band_r + r / p.Y; y = r % p.Y; }
+        else {
+            r -= nr;
+            const int nb = p.band_b * mid;
+            if (r < nb) { y = r / mid; x = p.band_l + r % mid; }
+            else { r -= nb; y = p.Y - p.band_t + r / mid; x = p.band_l + r % mid; }
+        }
+    }
+    // walk the chain backwards through S, W, N, E (application order E, N, W, S); slot k
+    // records where side order[k] acts, so the unrolled loops index registers only
+    int kx[4], ky[4];
+    bool applied[4];
+    int cx = x, cy = y;
+    bool have_source = true, stop = false, mirrored = false;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+        const int sd = k == 0 ? kEast : (k == 1 ? kNorth : (k == 2 ? kWest : kSouth));
+        applied[k] = !stop && in_strip(p, sd, cx, cy);
+        kx[k] = cx; ky[k] = cy;
+        if (applied[k]) {
+            const int ty = p.side[sd].type;
+            if (ty == kSolid) {
+                mirrored = true;
+                if (sd == kWest) cx = 4 - cx;
+                else if (sd == kSouth) cy = 4 - cy;
+                else if (sd == kEast) cx = (2 * p.nx + 2) - cx;
+                else cy = (2 * p.ny + 2) - cy;
+            } else if (ty != kSponge) {
+                have_source = false;
+                stop = true;
+            }
+        }
+    }
+    if ((int)mirrored != p.mirror_pass) return;
+    Val v{0.f, 0.f, 0.f};
+    if (have_source && cx >= 0 && cx < p.X && cy >= 0 && cy < p.Y) v = pre_value(p, cx, cy);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!applied[k]) continue;
+        const int sd = k == 0 ? kEast : (k == 1 ? kNorth : (k == 2 ? kWest : kSouth));
+        const SideParams &S = p.side[sd];
+        const int ox = kx[k], oy = ky[k];
+        switch (S.type) {
+        case kSolid:
+            if (sd == kWest || sd == kEast) v.hu = -v.hu; else v.hv = -v.hv;
+            break;
+        case kSponge: {
+            const float gamma = S.tab[strip_index(p, sd, ox, oy)];
+            v.w = gamma * (v.w - S.sl) + S.sl;
+            v.hu = gamma * v.hu;
+            v.hv = gamma * v.hv;
+            break;
+        }
+        case kSine: {
+            const float *tb = S.tab + (size_t)strip_index(p, sd, ox, oy) * 5;
+            float e = 0.f, hu = 0.f, hv = 0.f;
+            if (tb[4] != 0.f) {
+                const float arg = S.omega * p.t - tb[0] - tb[1] + 0.f;
+                e = S.amp * sinf(arg);
+                hu = tb[2] * e;
+                hv = tb[3] * e;
+            }
+            v.w = e + S.sl; v.hu = hu; v.hv = hv;
+            break;
+        }
+        case kUTS:
+            v.w = S.uts_eta + S.sl; v.hu = S.uts_hu; v.hv = S.uts_hv;
+            break;
+        default: {   // irregular
+            const float *q = S.irr + (size_t)strip_index(p, sd, ox, oy) * 3;
+            v.w = q[0]; v.hu = q[1]; v.hv = q[2];
+            break;
+        }
+        }
+    }
+    p.w[(size_t)y * p.P + x] = v.w;
+    p.hu[(size_t)y * p.P + x] = v.hu;
+    p.hvT[(size_t)x * p.PT + y] = v.hv;
+}
+
+// ------------------------------------------------------------------ utilities
+
+// out[j * pout + i] = in[i * pin + j] for i < rows, j < cols (host-side layout changes)
+__global__ void transpose_plane(const float *in, int pin, float *out, int pout, int rows, int cols)
+{
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r = by + j, c = bx + threadIdx.x;
+        if (r < rows && c < cols) tile[j][threadIdx.x] = in[(size_t)r * pin + c];
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int c = bx + j, r = by + threadIdx.x;
+        if (r < rows && c < cols) out[(size_t)c * pout + r] = tile[threadIdx.x][j];
+    }
+}
+
+// level-0 coefficients of every cell for the x- and y-lines (diagnostic export)
+__global__ void export_coefficients(const LineParams p, const float *BA, const float *BAT, float *ax, float *ay)
+{
+    const int X = p.nx + 4, Y = p.ny + 4;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= X * Y) return;
+    const int x = i % X, y = i / X;
+    const size_t N = (size_t)X * Y;
+    const float cs = p.coef_sl;
+    float A = 0.f, B = 1.f, C = 0.f;
+    if (x >= 2 && x <= X - 3) {
+        const int yy = y < 2 ? 2 : (y > p.ny + 1 ? p.ny + 1 : y);
+        const float *r = BA + (size_t)yy * p.P;
+        line_coef(p.kcoef, p.dx2, fmaxr(0.f, cs - r[x - 2]), fmaxr(0.f, cs - r[x - 1]), fmaxr(0.f, cs - r[x]),
+                  fmaxr(0.f, cs - r[x + 1]), fmaxr(0.f, cs - r[x + 2]), x == 2, x == X - 3, A, B, C);
+    }
+    ax[i] = A; ax[N + i] = B; ax[2 * N + i] = C;
+    A = 0.f; B = 1.f; C = 0.f;
+    if (y >= 2 && y <= Y - 3) {
+        const int xx = x < 2 ? 2 : (x > p.nx + 1 ? p.nx + 1 : x);
+        const float *cl = BAT + (size_t)xx * p.PT;
+        line_coef(p.kcoef, p.dy2, fmaxr(0.f, cs - cl[y - 2]), fmaxr(0.f, cs - cl[y - 1]), fmaxr(0.f, cs - cl[y]),
+                  fmaxr(0.f, cs - cl[y + 1]), fmaxr(0.f, cs - cl[y + 2]), y == 2, y == Y - 3, A, B, C);
+    }
+    ay[i] = A; ay[N + i] = B; ay[2 * N + i] = C;
+}
+
+template __global__ void explicit_step<0>(const ExplicitParams);
+template __global__ void explicit_step<1>(const ExplicitParams);
+template __global__ void explicit_step<2>(const ExplicitParams);
+template __global__ void explicit_step<3>(const ExplicitParams);
+
+}  // namespace cel
+
+// ------------------------------------------------------------------ launchers (host)
+
+namespace cel {
+
+hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st)
+{
+    const dim3 grid(p.nstrips * p.nbands), block(64);
+    switch (scheme) {
+    case 0: hipLaunchKernelGGL(explicit_step<0>, grid, block, 0, st, p); break;
+    case 1: hipLaunchKernelGGL(explicit_step<1>, grid, block, 0, st, p); break;
+    case 2: hipLaunchKernelGGL(explicit_step<2>, grid, block, 0, st, p); break;
+    default: hipLaunchKernelGGL(explicit_step<3>, grid, block, 0, st, p); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_t st)
+{
+    static bool attr_set = false;
+    const size_t lds = sizeof(float) * 4 * (size_t)(p.nx > p.ny ? p.nx : p.ny);
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(solve_lines),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 4 * 4 * CEL_MAX_LINE_);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    int irr_lines = 0;
+    for (int s = 0; s < 4; ++s) irr_lines += irr.count[s];
+    const int irr_blocks = (irr_lines + (kSolveThreads / 64) - 1) / (kSolveThreads / 64);
+    const dim3 grid(p.nlines_x + p.nlines_y + irr_blocks), block(kSolveThreads);
+    hipLaunchKernelGGL(solve_lines, grid, block, lds, st, p, irr);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
+{
+    if (p.ncells == 0) return hipSuccess;
+    const dim3 grid((p.ncells + 255) / 256), block(256);
+    hipLaunchKernelGGL(boundary_apply, grid, block, 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_coefficients(const LineParams &p, const float *BA, const float *BAT, float *ax, float *ay,
+                                      hipStream_t st)
+{
+    const int n = (p.nx + 4) * (p.ny + 4);
+    hipLaunchKernelGGL(export_coefficients, dim3((n + 255) / 256), dim3(256), 0, st, p, BA, BAT, ax, ay);
+    return hipGetLastError();
+}
+
+}  // namespace cel

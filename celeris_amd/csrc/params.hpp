@@ -71,6 +71,12 @@ struct IrregularSide {
     float *out;                          // [(strip * nlines + line) * 3]
 };
 
+struct IrregularAll {
+    IrregularSide side[4];
+    int count[4];      // width * nlines per side (0 if inactive)
+    float t;
+};
+
 struct SideParams {
     int type, width;
     float sl;

@@ -145,8 +145,9 @@ int cel_engine_scheme(cel_engine *e, int *scheme);
  * on the engine's stream (inputs are already resident in HBM). */
 int cel_engine_timed_steps(cel_engine *e, int nsteps, float *ms);
 
-/* Per-kernel device time (ms) accumulated by cel_engine_timed_steps when profiling is
- * on; names: "explicit", "solve_x", "solve_y", "boundary". */
+/* Per-kernel device time (ms) accumulated by the steps run while profiling is on
+ * (profiling synchronises after every step); names: "explicit_step", "solve_lines",
+ * "boundary_apply". */
 int cel_engine_set_profiling(cel_engine *e, int on);
 int cel_engine_kernel_time(cel_engine *e, const char *name, float *ms, int *launches);
 
