@@ -70,6 +70,7 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
         if (bnd[i].type < CEL_SOLID || bnd[i].type > CEL_UNIFORMTIMESERIES) throw Error(CEL_EINVAL, "bad boundary type");
         side_[i].cfg = bnd[i];
     }
+    current_timestep_ = old_timestep_ = old_old_timestep_ = s.timestep;
     check(hipGetDevice(&device_), "hipGetDevice");
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");

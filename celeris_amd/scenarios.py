@@ -70,14 +70,21 @@ def _finish(name, s, bnd, z, steps, solitons=(), info=None):
     return Scenario(name, s, bnd, bottom, state, steps, info or {})
 
 
-def cfl_dt(dx, hmax, cfl=0.5):
+# The AB3 predictor of the reference scheme is stable only for small Courant numbers: on a
+# flat 1 m basin with a soliton it stays bounded at CFL 0.12 and diverges at 0.15 (checked
+# with the CPU oracle).  The reference's own example cases run at CFL ~0.09
+# (e.g. conical island: dx 0.1 m, dt 0.005 s, h 0.32 m), so the scenarios use 0.1.
+DEFAULT_CFL = 0.1
+
+
+def cfl_dt(dx, hmax, cfl=DEFAULT_CFL):
     return float(np.float32(cfl * dx / np.sqrt(G * hmax)))
 
 
 def plain_beach(nx=256, ny=256, dx=0.5, steps=2000):
     """Flat 1.0 m depth for x < 60 % of the domain, then a 1:20 slope up to a 0.3 m dry beach
     plateau; west sine wave (a 0.05 m, T 2 s), north/south walls, 20-cell east sponge,
-    Bcoef 1/15, minmod theta 1.3, CFL 0.5 on the 1 m depth."""
+    Bcoef 1/15, minmod theta 1.3, dt from CFL 0.1 on the 1 m depth."""
     W, L = dx * (nx - 1), dx * (ny - 1)
     X, _ = corner_coords(nx, ny, W, L)
     z = np.where(X < 0.6 * W, -1.0, np.minimum(0.3, -1.0 + (X - 0.6 * W) / 20.0))
@@ -125,7 +132,7 @@ def small(nx=48, ny=40, dx=0.5, seed=0, bnd=None, soliton=True, steps=100, corre
     X, Y = corner_coords(nx, ny, W, L)
     z = -0.8 + 1.0 * np.clip((X - 0.55 * W) / (0.45 * W), 0, None) ** 1.5
     z += 0.08 * np.sin(2 * np.pi * Y / (0.7 * L + 1.0)) + gaussian_random_field(X.shape, 3.0, 0.04, rng)
-    s = settings(nx, ny, W, L, cfl_dt(dx, 1.0, 0.4), theta=1.3, correction_steps_num=correction_steps,
+    s = settings(nx, ny, W, L, cfl_dt(dx, 1.0), theta=1.3, correction_steps_num=correction_steps,
                  time_scheme=time_scheme, friction=friction, is_manning=is_manning,
                  still_water_elevation=still_water, is_boussinesq=is_boussinesq)
     if bnd is None:

@@ -1,0 +1,333 @@
+"""Pins for the CPU oracle (no GPU): known answers of the compute.hlsl helpers, the
+tridiagonal solver against a dense solve, invariants of the whole step (lake at rest, mass
+conservation, transpose and mirror symmetry) and physics closed forms (Madsen-Sorensen
+dispersion relation, solitary-wave celerity).  Each check is chosen so that a plausible
+transcription mistake (a dropped term, a wrong sign, a swapped index) fails it.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from celeris_amd import SINEWAVE, SOLID, SPONGE, Boundary, bottom_from_corners, initial_state, settings
+from celeris_amd import scenarios as S
+from oracle import binding as O
+
+B = 1.0 / 15.0
+G = 9.81
+
+
+def f(x):
+    return ctypes.c_float(x)
+
+
+# ---------------------------------------------------------------- helper KATs
+
+def test_minmod_signs():
+    L = O.lib()
+    assert L.orc_f32_minmod(1.0, 2.0, 3.0) == 1.0
+    assert L.orc_f32_minmod(-1.0, -2.0, -0.5) == -0.5
+    assert L.orc_f32_minmod(-1.0, 2.0, 3.0) == 0.0
+    assert L.orc_f32_minmod(0.0, 2.0, 3.0) == 0.0
+
+
+def test_reconstruct_exact_on_linear_data():
+    L = O.lib()
+    w, e = ctypes.c_float(), ctypes.c_float()
+    for theta in (1.0, 1.3, 2.0):
+        # u = 3 + 0.5 j: face values at j +- 1/2 are exact for theta >= 1
+        L.orc_f32_reconstruct(2 * theta, 2.5, 3.0, 3.5, ctypes.byref(w), ctypes.byref(e))
+        assert (w.value, e.value) == (2.75, 3.25)
+    # extremum: slope limited to zero
+    L.orc_f32_reconstruct(2.6, 1.0, 2.0, 1.5, ctypes.byref(w), ctypes.byref(e))
+    assert w.value == e.value == 2.0
+
+
+def test_correct_w_positivity():
+    L = O.lib()
+    ww, we = ctypes.c_float(0.2), ctypes.c_float(-0.3)
+    L.orc_f32_correct_w(-0.1, 0.0, 0.05, ctypes.byref(ww), ctypes.byref(we))
+    assert we.value == 0.0 and ww.value == pytest.approx(max(-0.1, 2 * 0.05 - 0.0))
+    assert ww.value + we.value == pytest.approx(2 * 0.05)         # mean preserved
+    ww, we = ctypes.c_float(-0.3), ctypes.c_float(0.2)
+    L.orc_f32_correct_w(0.0, -0.1, 0.05, ctypes.byref(ww), ctypes.byref(we))
+    assert ww.value == 0.0 and we.value == pytest.approx(0.1)
+
+
+def test_desingularised_velocity():
+    L = O.lib()
+    for h in (0.5, 1.0, 3.0):
+        assert L.orc_f32_divide_by_h(h, 5e-8) == pytest.approx(1.0 / h, rel=1e-6)
+    assert L.orc_f32_divide_by_h(0.0, 5e-8) == 0.0
+    assert L.orc_f32_divide_by_h(1e-4, 5e-8) < 1.0                 # bounded near dry
+
+
+def test_numerical_flux_limits():
+    L = O.lib()
+    assert L.orc_f32_numerical_flux(1.0, 1.0, 5.0, 7.0, 2.0) == 0.0
+    # supersonic flow to the right (a- = 0): upwind flux F(U-)
+    assert L.orc_f32_numerical_flux(2.0, 0.0, 5.0, 7.0, 3.0) == pytest.approx(7.0)
+    # to the left (a+ = 0): F(U+)
+    assert L.orc_f32_numerical_flux(0.0, -2.0, 5.0, 7.0, 3.0) == pytest.approx(5.0)
+    # consistency: equal states give the physical flux
+    assert L.orc_f32_numerical_flux(3.0, -1.5, 4.0, 4.0, 0.0) == pytest.approx(4.0)
+
+
+def test_sponge_profile():
+    L = O.lib()
+    assert L.orc_f32_sponge(10.0, 0.0) == pytest.approx(0.0, abs=1e-7)
+    assert L.orc_f32_sponge(10.0, 10.0) == 1.0
+    assert L.orc_f32_sponge(10.0, 25.0) == 1.0
+    assert L.orc_f32_sponge(10.0, 5.0) == pytest.approx(0.5, abs=1e-6)
+
+
+@pytest.mark.parametrize("T,d", [(2.0, 1.0), (6.0, 8.0), (1.0, 0.3), (3.0, 2.0)])
+def test_eckart_wavenumber_vs_exact_dispersion(T, d):
+    w = 2 * np.pi / T
+    k = O.lib().orc_f32_wavenumber(w, d, G)
+    k_exact = 1.0
+    for _ in range(200):                                           # w^2 = g k tanh(k d)
+        k_exact = w * w / (G * np.tanh(k_exact * d))
+    assert abs(k - k_exact) / k_exact < 0.06                       # Eckart's approximation error
+
+
+def test_adams_bashforth_coefficients():
+    c = np.zeros(3, np.float32)
+    O.lib().orc_f32_ab_coef(0.1, 0.1, 0.1, O.fptr(c))
+    np.testing.assert_allclose(c / 6.0, np.array([23, -16, 5]) / 12.0, rtol=1e-6)
+    # variable steps: the scheme integrates y' = t^2 exactly (third order)
+    dt, d1, d2 = 0.3, 0.2, 0.45
+    O.lib().orc_f32_ab_coef(dt, d1, d2, O.fptr(c))
+    t0 = 1.0
+    f_n, f_1, f_2 = t0 ** 2, (t0 - d1) ** 2, (t0 - d1 - d2) ** 2
+    step = dt / 6.0 * (c[0] * f_n + c[1] * f_1 + c[2] * f_2)
+    exact = ((t0 + dt) ** 3 - t0 ** 3) / 3.0
+    assert step == pytest.approx(exact, rel=1e-5)
+
+
+# ---------------------------------------------------------------- tridiagonal
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 7, 8, 13, 64, 100, 301, 1024, 2047])
+def test_cyclic_reduction_matches_dense_solve(n):
+    rng = np.random.default_rng(n)
+    a = rng.uniform(-1, 0, n); c = rng.uniform(-1, 0, n)
+    b = 2.2 + rng.uniform(0, 1, n); d = rng.standard_normal(n)
+    a[0] = 0; c[-1] = 0
+    M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    ref = np.linalg.solve(M, d)
+    np.testing.assert_allclose(O.cr_solve(a, b, c, d, "f64"), ref, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(O.cr_solve(a, b, c, d, "f32"), ref, rtol=0, atol=2e-6)
+
+
+def test_level0_coefficients_closed_form():
+    sc = S.small(nx=30, ny=26, seed=2)
+    o = O.Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f64")
+    ax, ay = o.coefficients()
+    BA = sc.bottom[2].astype(np.float64)
+    d = np.maximum(0.0, 0.0 - BA)
+    dx = sc.s.width / (sc.s.nx - 1)
+    y, x = 10, 12
+    K1 = (B + 1 / 3) * d[y, x] ** 2 / dx ** 2
+    dd = (-d[y, x + 2] + 8 * d[y, x + 1] - 8 * d[y, x - 1] + d[y, x - 2]) / (12 * dx)   # d'(x)
+    # A = -K1 + (d/dx^2)*(12 dx d')/72 = -(B+1/3) d^2/dx^2 + d d'/(6 dx)
+    assert ax[0, y, x] == pytest.approx(-K1 + d[y, x] * dd / (6 * dx), rel=1e-6)
+    assert ax[1, y, x] == pytest.approx(1 + 2 * K1, rel=1e-9)
+    assert ax[2, y, x] == pytest.approx(-K1 - d[y, x] * dd / (6 * dx), rel=1e-6)
+    assert ax[0, y, 2] == 0 and ax[2, y, sc.s.nx + 1] == 0          # open ends of every line
+    assert ay[0, 2, x] == 0 and ay[2, sc.s.ny + 1, x] == 0
+    assert (ax[:, :, :2] == np.array([0, 1, 0])[:, None, None]).all()   # (0,1,0) margins
+
+
+# ---------------------------------------------------------------- preprocessing
+
+def test_bottom_from_corners_matches_oracle_and_closed_form():
+    nx, ny = 21, 17
+    rng = np.random.default_rng(1)
+    z = rng.standard_normal((ny + 5, nx + 5)).astype(np.float32)
+    prod = bottom_from_corners(nx, ny, z)
+    orc = O.bottom_from_corners(nx, ny, z)
+    assert np.array_equal(prod, orc)
+    z64 = z.astype(np.float64)
+    BA = 0.25 * (z64[:-1, :-1] + z64[:-1, 1:] + z64[1:, :-1] + z64[1:, 1:])
+    BX = 0.5 * (z64[:-1, 1:] + z64[1:, 1:])
+    BY = 0.5 * (z64[1:, :-1] + z64[1:, 1:])
+    np.testing.assert_allclose(prod, np.stack([BY, BX, BA]), atol=1e-6)
+
+
+def test_initial_state_lake_and_soliton():
+    sc = S.small(nx=40, ny=30, soliton=False)
+    st = sc.state
+    BA = sc.bottom[2]
+    X, Y = sc.s.nx + 4, sc.s.ny + 4
+    ii = np.clip(np.arange(X), 2, sc.s.nx + 1)
+    jj = np.clip(np.arange(Y), 2, sc.s.ny + 1)
+    np.testing.assert_array_equal(st[0], np.maximum(0.0, BA[np.ix_(jj, ii)]))
+    assert not st[1].any() and not st[2].any()
+    # flat 1 m basin, soliton H = 0.05 at xc: eta = H sech^2(K (x - xc)), hu = c eta
+    nx, ny, dx = 80, 10, 0.5
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    s = settings(nx, ny, W, L, 0.01)
+    b = bottom_from_corners(nx, ny, np.full((ny + 5, nx + 5), -1.0, np.float32))
+    st = initial_state(s, b, [(0.05, 0.0, 1e6, 20.0, 0.0)])
+    x = dx * np.arange(nx + 4)
+    K = np.sqrt(0.75 * 0.05 / 1.0)
+    eta = 0.05 / np.cosh(K * (x - 20.0)) ** 2
+    np.testing.assert_allclose(st[0, 5], eta, atol=2e-7)
+    np.testing.assert_allclose(st[1, 5], np.sqrt(G * 1.05) * eta, rtol=1e-5, atol=1e-8)
+
+
+# ---------------------------------------------------------------- step invariants
+
+def _wet_basin(nx=40, ny=34, dx=0.5, seed=0, soliton=True):
+    rng = np.random.default_rng(seed)
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    X, Y = S.corner_coords(nx, ny, W, L)
+    z = -1.0 + 0.25 * np.sin(X / 3.0) * np.cos(Y / 2.5) + S.gaussian_random_field(X.shape, 3.0, 0.05, rng)
+    # dyadic corner heights: the face and centre means are then exact in FP32, so the
+    # hydrostatic balance BA = (BE + BW) / 2 holds exactly and rest stays rest to round-off
+    z = np.round(z * 256.0) / 256.0
+    s = settings(nx, ny, W, L, S.cfl_dt(dx, 1.4))
+    bottom = bottom_from_corners(nx, ny, z.astype(np.float32))
+    sol = [(0.08, 0.4, 1e6, 0.35 * W, 0.5 * L)] if soliton else []
+    return s, [Boundary(SOLID, 2) for _ in range(4)], bottom, initial_state(s, bottom, sol)
+
+
+def test_lake_at_rest_is_preserved():
+    s, bnd, bottom, state = _wet_basin(soliton=False)
+    for prec, tol in (("f64", 1e-12), ("f32", 2e-6)):
+        o = O.Oracle(s, bnd, bottom, state, prec)
+        o.step(200)
+        st = o.state()
+        assert np.abs(st[0, 2:-2, 2:-2]).max() < tol
+        assert np.abs(st[1:, 2:-2, 2:-2]).max() < tol
+
+
+def test_mass_conserved_in_a_closed_lake():
+    # Every update of w is a difference of face fluxes (compute.hlsl:718-721), so the
+    # volume sum(w - BA) over the interior changes only through the outermost faces.  In a
+    # bowl whose rim is dry land those faces carry no water and the volume is constant to
+    # round-off.  (With water at a solid wall the reference is not exactly conservative:
+    # the wall cell's normal momentum is not forced to zero by the mirror boundary.)
+    nx, ny, dx = 48, 40, 0.5
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    X, Y = S.corner_coords(nx, ny, W, L)
+    R = 0.5 * min(W, L)
+    r2 = ((X - W / 2) ** 2 + (Y - L / 2) ** 2) / R ** 2
+    z = -1.0 + 1.5 * r2 + 0.05 * np.sin(X / 2.0) * np.cos(Y / 3.0)
+    s = settings(nx, ny, W, L, S.cfl_dt(dx, 1.2))
+    bottom = bottom_from_corners(nx, ny, z.astype(np.float32))
+    BA = bottom[2].astype(np.float64)
+    state = initial_state(s, bottom)
+    xc = dx * (np.arange(nx + 4) - 2)
+    yc = dx * (np.arange(ny + 4) - 2)
+    hump = 0.06 * np.exp(-((xc[None, :] - 0.4 * W) ** 2 + (yc[:, None] - 0.5 * L) ** 2) / 4.0)
+    wet = state[0] > bottom[2]
+    state[0] = np.where(wet, state[0] + hump, state[0]).astype(np.float32)
+    o = O.Oracle(s, [Boundary(SOLID, 2) for _ in range(4)], bottom, state, "f64")
+    m0 = (state[0].astype(np.float64) - BA)[2:-2, 2:-2].sum()
+    o.step(400)
+    st = o.state()
+    m1 = (st[0] - BA)[2:-2, 2:-2].sum()
+    assert abs(m1 - m0) / m0 < 1e-12
+    assert np.abs(st[1]).max() > 1e-3                               # the water did move
+    assert np.abs((st[0] - BA)[2:4, 2:-2]).max() < 1e-6             # and the rim stayed dry
+
+
+def test_transpose_symmetry():
+    nx = ny = 36
+    dx = 0.5
+    W = dx * (nx - 1)
+    Xc, Yc = S.corner_coords(nx, ny, W, W)
+    z = -1.0 + 0.2 * np.sin(Xc / 2.7) + 0.15 * np.cos(Yc / 3.1) + 0.1 * np.sin((Xc + 2 * Yc) / 4.0)
+    s = settings(nx, ny, W, W, S.cfl_dt(dx, 1.5))
+    bnd = [Boundary(SOLID, 2) for _ in range(4)]
+    b1 = bottom_from_corners(nx, ny, z.astype(np.float32))
+    b2 = bottom_from_corners(nx, ny, z.T.copy().astype(np.float32))
+    st1 = initial_state(s, b1, [(0.08, 0.3, 1e6, 7.0, 9.0)])
+    st2 = initial_state(s, b2, [(0.08, np.pi / 2 - 0.3, 1e6, 9.0, 7.0)])
+    o1 = O.Oracle(s, bnd, b1, st1, "f64"); o1.step(200)
+    o2 = O.Oracle(s, bnd, b2, st2, "f64"); o2.step(200)
+    a, c = o1.state(), o2.state()
+    # (w, hu, hv)(x, y) == (w, hv, hu)(y, x)
+    np.testing.assert_allclose(a[0], c[0].T, atol=5e-7)
+    np.testing.assert_allclose(a[1], c[2].T, atol=5e-7)
+    np.testing.assert_allclose(a[2], c[1].T, atol=5e-7)
+
+
+def test_mirror_symmetry_of_centred_soliton():
+    nx, ny, dx = 60, 41, 0.5
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    Xc, Yc = S.corner_coords(nx, ny, W, L)
+    z = -1.0 + 0.3 * np.cos(np.pi * (Yc - L / 2) / L) * np.sin(Xc / 5.0)
+    z = 0.5 * (z + z[::-1])                                                     # exactly symmetric about y = L/2
+    s = settings(nx, ny, W, L, S.cfl_dt(dx, 1.4))
+    b = bottom_from_corners(nx, ny, z.astype(np.float32))
+    st = initial_state(s, b, [(0.08, 0.0, 1e6, 0.3 * W, 0.0)])
+    # the IC uses y = j*dy, so centre the crest on the mirror row: y-independent here
+    o = O.Oracle(s, [Boundary(SOLID, 2) for _ in range(4)], b, st, "f64")
+    o.step(250)
+    a = o.state()[:, 2:-2, 2:-2]
+    np.testing.assert_allclose(a[0], a[0][::-1], atol=1e-7)
+    np.testing.assert_allclose(a[1], a[1][::-1], atol=1e-7)
+    np.testing.assert_allclose(a[2], -a[2][::-1], atol=1e-7)
+
+
+# ---------------------------------------------------------------- physics closed forms
+
+def _channel_wavenumber(T, h=1.0, dx=0.1, nx=600, steps=8000):
+    ny = 6
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    s = settings(nx, ny, W, L, S.cfl_dt(dx, h))
+    z = np.full((ny + 5, nx + 5), -h, np.float32)
+    b = bottom_from_corners(nx, ny, z)
+    bnd = [Boundary(SINEWAVE, 2, amplitude=0.002, period=T), Boundary(SPONGE, 150), Boundary(SOLID, 2),
+           Boundary(SOLID, 2)]
+    o = O.Oracle(s, bnd, b, initial_state(s, b), "f64")
+    o.step(steps)
+    eta = o.state()[0, 4, 2:-2]
+    x = dx * np.arange(nx)
+    sel = (x > 3.0) & (x < 20.0)            # established wave train, away from front and sponge
+    xs, es = x[sel], eta[sel]
+    zc = xs[:-1][np.sign(es[:-1]) != np.sign(es[1:])]
+    # interpolate crossings
+    i = np.nonzero(np.sign(es[:-1]) != np.sign(es[1:]))[0]
+    xz = xs[i] - es[i] * (xs[i + 1] - xs[i]) / (es[i + 1] - es[i])
+    half = np.diff(xz)
+    return np.pi / np.median(half), len(zc)
+
+
+def test_madsen_sorensen_dispersion_relation():
+    T, h = 1.6, 1.0
+    w = 2 * np.pi / T
+    k_meas, n = _channel_wavenumber(T, h)
+    assert n >= 6
+    # omega^2 = g h k^2 (1 + B (kh)^2) / (1 + (B + 1/3)(kh)^2)
+    k_ms = 1.0
+    for _ in range(200):
+        kh = k_ms * h
+        k_ms = w / np.sqrt(G * h * (1 + B * kh ** 2) / (1 + (B + 1 / 3) * kh ** 2))
+    k_sw = w / np.sqrt(G * h)
+    assert abs(k_meas - k_ms) / k_ms < 0.01, (k_meas, k_ms, k_sw)
+    assert abs(k_meas - k_sw) / k_sw > 0.15                         # not the shallow-water speed
+
+
+def test_solitary_wave_celerity():
+    nx, ny, dx, h, H = 500, 6, 0.1, 1.0, 0.1
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    s = settings(nx, ny, W, L, S.cfl_dt(dx, h + H))
+    b = bottom_from_corners(nx, ny, np.full((ny + 5, nx + 5), -h, np.float32))
+    st = initial_state(s, b, [(H, 0.0, 1e6, 12.0, 0.0)])
+    o = O.Oracle(s, [Boundary(SOLID, 2) for _ in range(4)], b, st, "f64")
+    x = dx * np.arange(nx + 4)
+
+    def crest():
+        e = o.state()[0, 4]
+        i = int(np.argmax(e))
+        a, m, c = e[i - 1], e[i], e[i + 1]
+        return x[i] + 0.5 * dx * (a - c) / (a - 2 * m + c)
+
+    o.step(200)
+    t0, x0 = o.time, crest()
+    o.step(1500)
+    c_meas = (crest() - x0) / (o.time - t0)
+    assert c_meas == pytest.approx(np.sqrt(G * (h + H)), rel=0.03)
