@@ -1,0 +1,122 @@
+"""Benchmark of the Celeris Boussinesq step on one MI355X.
+
+Workload: 2048 x 2048 interior cells, dx = 1 m, 1:30 plane from 8 m depth to 2 m dry land
+plus a seeded Gaussian random field (std 0.3 m, correlation 20 cells), west irregular
+waves (100 JONSWAP components, Hs 0.1 m, Tp 3 s, gamma 3.3, +-30 deg), 20-cell east
+sponge, solid north/south walls, FP32.  A step is one full Boussinesq time step (explicit
+stencil, x- and y-line tridiagonal solves, boundaries) over the whole grid; inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line: Gcell-steps/s (interior cells x steps / wall time), the dominant
+kernel's achieved HBM bandwidth against MI355X's 8 TB/s, and the CPU oracle timed on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_CELL_STEP = 96          # algorithmic HBM bytes per interior cell-step (plain FP32)
+HBM_PEAK_GBS = 8000.0             # MI355X HBM3E peak
+
+
+def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
+    """FP32 CPU oracle (OpenMP over rows) on the same 2048^2 workload for a bounded number
+    of steps; returns Gcell-steps/s and the sample description."""
+    from oracle.binding import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
+    t0 = time.perf_counter()
+    o.step(2)                                     # warm-up (Euler bootstrap steps)
+    t1 = time.perf_counter()
+    per = max((t1 - t0) / 2, 1e-6)
+    n = int(max(3, min(max_steps, seconds_target / per)))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    cells = sc.s.nx * sc.s.ny
+    return {"value": cells * n / dt / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle, same {sc.s.nx}x{sc.s.ny} workload, {n} steps after 2 warm-up steps, "
+                      f"{dt:.1f} s, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding of the step is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from celeris_amd import ShallowWaterEngine
+    from celeris_amd import scenarios as S
+
+    sc = S.headline(nx=args.nx, ny=args.ny)
+    eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    cells = sc.s.nx * sc.s.ny
+    eng.timestep(args.warmup)
+    eng.sync()
+
+    # timed region: K steps bracketed by device synchronisation, wall clock and HIP events
+    t0 = time.perf_counter()
+    ms_events = eng.timed_steps(args.steps)
+    wall = time.perf_counter() - t0
+    value = cells * args.steps / wall / 1e9
+
+    # the same number of steps again with per-kernel HIP events on the engine's stream
+    eng.set_profiling(True)
+    eng.timed_steps(args.steps)
+    kern = {}
+    for name in ("explicit_step", "solve_lines", "boundary_apply"):
+        ms, n = eng.kernel_time(name)
+        kern[name] = ms / max(n, 1)
+    eng.set_profiling(False)
+    st = eng.get_state()
+    finite = bool(np.isfinite(st).all())
+    eng.close()
+
+    avg_ms = kern["explicit_step"]
+    achieved = BYTES_PER_CELL_STEP * cells / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-steps/s (2048^2 Boussinesq)",
+        "value": value,
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Boussinesq step, {sc.s.nx}x{sc.s.ny} interior cells, dx 1 m, 1:30 slope + GRF, "
+                               "west irregular waves (100 JONSWAP components), 20-cell east sponge, N/S walls, FP32",
+                   "nx": sc.s.nx, "ny": sc.s.ny, "dt": sc.s.timestep,
+                   "hbm_frac_step": value * 1e9 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
+                   "event_ms_per_step": ms_events / args.steps,
+                   "kernel_avg_ms": kern, "state_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "explicit_step",
+                     "bytes_per_launch": BYTES_PER_CELL_STEP * cells},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

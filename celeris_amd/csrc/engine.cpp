@@ -86,6 +86,7 @@ ShallowWaterEngine::~ShallowWaterEngine()
     for (int i = 0; i < 4; ++i) freeSideTables(i);
     if (arena_) (void)hipFree(arena_);
     for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
+    for (auto &e : pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -503,9 +504,13 @@ void ShallowWaterEngine::timestep()
     const int n = 1 - cur_;
     ExplicitParams ep;
     fillExplicit(ep);
-    if (profiling_) check(hipEventRecord(ev_[0], stream_), "event");
+    hipEvent_t pe[4] = {};
+    if (profiling_) {
+        for (auto &e : pe) e = nextEvent();
+        check(hipEventRecord(pe[0], stream_), "event");
+    }
     check(launch_explicit(scheme, ep, stream_), "explicit_step");
-    if (profiling_) check(hipEventRecord(ev_[1], stream_), "event");
+    if (profiling_) check(hipEventRecord(pe[1], stream_), "event");
     // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
     // steps nothing ever reads them
     if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
@@ -547,8 +552,9 @@ void ShallowWaterEngine::timestep()
         if (sd.irr_nw == 0) S.nw = 1;
         irr.count[s] = S.width * S.nlines;
     }
+    if (profiling_) check(hipEventRecord(pe[2], stream_), "event");
     check(launch_solve(lp, irr, stream_), "solve_lines");
-    if (profiling_) check(hipEventRecord(ev_[2], stream_), "event");
+    if (profiling_) check(hipEventRecord(pe[3], stream_), "event");
     // boundaries (engine.cpp:1235-1438), boundary time = t^n (engine.cpp:1324)
     BoundaryParams bp;
     std::memset(&bp, 0, sizeof(bp));
@@ -573,12 +579,8 @@ void ShallowWaterEngine::timestep()
     bp.mirror_pass = 0;
     check(launch_boundary(bp, stream_), "boundary_apply");
     if (profiling_) {
-        check(hipEventRecord(ev_[3], stream_), "event");
-        check(hipEventSynchronize(ev_[3]), "event sync");
-        float ms;
-        check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event"); kt_[0].ms += ms; kt_[0].n++;
-        check(hipEventElapsedTime(&ms, ev_[1], ev_[2]), "event"); kt_[1].ms += ms; kt_[1].n++;
-        check(hipEventElapsedTime(&ms, ev_[2], ev_[3]), "event"); kt_[2].ms += ms; kt_[2].n++;
+        hipEvent_t e4 = nextEvent();
+        check(hipEventRecord(e4, stream_), "event");
     }
     // time-scheme state machine for the next step (engine.cpp:1476-1521)
     if (timestep_count_ >= 2) {
@@ -619,11 +621,45 @@ float ShallowWaterEngine::timedSteps(int nsteps)
     check(hipEventSynchronize(ev_[5]), "event sync");
     float ms = 0.f;
     check(hipEventElapsedTime(&ms, ev_[4], ev_[5]), "event");
+    if (profiling_) collectProfile();
     return ms;
 }
 
-bool ShallowWaterEngine::kernelTime(const std::string &name, float *ms, int *launches) const
+hipEvent_t ShallowWaterEngine::nextEvent()
 {
+    if (pool_used_ == pool_.size()) {
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        pool_.push_back(e);
+    }
+    return pool_[pool_used_++];
+}
+
+void ShallowWaterEngine::setProfiling(bool on)
+{
+    sync();
+    if (profiling_) collectProfile();
+    profiling_ = on;
+    for (auto &k : kt_) k = KTime{};
+}
+
+// Each profiled step recorded five events: before explicit_step, after it, before
+// solve_lines, after it, after the boundary launches.
+void ShallowWaterEngine::collectProfile()
+{
+    sync();
+    for (size_t i = 0; i + 5 <= pool_used_; i += 5) {
+        float ms;
+        check(hipEventElapsedTime(&ms, pool_[i], pool_[i + 1]), "event"); kt_[0].ms += ms; kt_[0].n++;
+        check(hipEventElapsedTime(&ms, pool_[i + 2], pool_[i + 3]), "event"); kt_[1].ms += ms; kt_[1].n++;
+        check(hipEventElapsedTime(&ms, pool_[i + 3], pool_[i + 4]), "event"); kt_[2].ms += ms; kt_[2].n++;
+    }
+    pool_used_ = 0;
+}
+
+bool ShallowWaterEngine::kernelTime(const std::string &name, float *ms, int *launches)
+{
+    if (profiling_) collectProfile();
     int i = -1;
     if (name == "explicit_step") i = 0;
     else if (name == "solve_lines") i = 1;

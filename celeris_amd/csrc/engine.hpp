@@ -63,8 +63,8 @@ public:
     double time() const { return total_time_; }
     int scheme() const { return time_scheme_now_; }
     float timedSteps(int n);
-    void setProfiling(bool on) { profiling_ = on; }
-    bool kernelTime(const std::string &name, float *ms, int *launches) const;
+    void setProfiling(bool on);
+    bool kernelTime(const std::string &name, float *ms, int *launches);
     size_t deviceBytes() const { return arena_bytes_ + table_bytes_; }
     void coefficients(float *ax, float *ay);
 
@@ -108,11 +108,16 @@ private:
     float total_time_ = 0.f;
     float current_timestep_, old_timestep_, old_old_timestep_;
 
-    // profiling
+    // profiling: five events per step recorded without host synchronisation and
+    // reduced after the stream drains (collectProfile)
     bool profiling_ = false;
     struct KTime { float ms = 0.f; int n = 0; };
-    KTime kt_[4];
+    KTime kt_[3];
     hipEvent_t ev_[8] = {};
+    std::vector<hipEvent_t> pool_;
+    size_t pool_used_ = 0;
+    hipEvent_t nextEvent();
+    void collectProfile();
 };
 
 }  // namespace cel
