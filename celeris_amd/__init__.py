@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libceleris_amd.so")
+LIB_PATH = os.environ.get("CELERIS_AMD_LIB", os.path.join(HERE, "libceleris_amd.so"))
 
 # BoundaryType (settings.hpp:90-96), BoundarySide, TimeIntegrationScheme (settings.hpp:304)
 SOLID, SPONGE, SINEWAVE, IRREGULARWAVES, UNIFORMTIMESERIES = range(5)

@@ -21,19 +21,19 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def build(verbose=False, out=LIB, defines=()):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc] + FLAGS + ["-x", "hip"] + srcs + ["-o", tmp]
+    tmp = out + ".tmp"
+    cmd = [hipcc] + FLAGS + [f"-D{d}" for d in defines] + ["-x", "hip"] + srcs + ["-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+    log = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + out)
-    if verbose and out.strip():
-        print(out)
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError("hipcc failed:\n" + log)
+    if verbose and log.strip():
+        print(log)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

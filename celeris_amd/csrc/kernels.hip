@@ -7,8 +7,9 @@
 //                     (60 produce output, 2 halo lanes each side) and exchange
 //                     horizontal neighbours through cross-lane moves, vertical
 //                     neighbours stay in registers.  (compute.hlsl Pass1/Pass2/Pass3)
-//   solve_lines     : one workgroup per x-line or y-line; cyclic reduction in LDS with
-//                     the reference's reduced coefficients, recomputed from BA.
+//   solve_lines_block : one workgroup per x-line or y-line; the reference's cyclic
+//                     reduction with each thread holding a chunk of the line in registers
+//                     and the upper levels in LDS; coefficients recomputed from BA.
 //                     (engine.cpp:2612-2887 + compute.hlsl:835-947)
 //   irregular_lines : irregular-wave superposition with the reference's pairwise-tree
 //                     sum order (compute.hlsl:1196-1309), merged into the solve launch.
@@ -20,7 +21,6 @@
 #include <hip/hip_runtime.h>
 #include "params.hpp"
 
-#define CEL_MAX_LINE_ 8192
 
 namespace cel {
 
@@ -151,14 +151,18 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 {
     constexpr int R = kBandRows;
     __shared__ float hvblk[R + 4][64];
-    __shared__ float dystg[R][64];
     __shared__ float chvblk[SCHEME == 2 ? (R + 4) : 1][64];
+    __shared__ float dystg[R][64];      // y-line right-hand sides, stored transposed after the band
 
     const int lane = threadIdx.x;
-    const int wid = blockIdx.x;
-    const int strip = wid % p.nstrips;
-    const int band = wid / p.nstrips;
-    if (band >= p.nbands) return;
+    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of
+    // tiles (strip-fastest) so neighbouring strips and bands share that XCD's L2
+    const int ntiles = p.nstrips * p.nbands;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int strip = tile % p.nstrips;
+    const int band = tile / p.nstrips;
     const int X = p.nx + 4;
     const int x0 = 2 + strip * kStripCols;
     const int x = x0 - 2 + lane;
@@ -401,7 +405,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
                 }
             }
             p.w_new[i] = wOut;
-            p.Dx[i] = huOut;
+            p.Dx[(size_t)y * P + (x - 2)] = huOut;
             dystg[y - ys][lane] = hvOut;
             p.Gs[0][i] = dr;
             p.Gs[1][i] = dg;
@@ -418,8 +422,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
         Ncur = N1;
         yf_s0 = yf0; yf_s1 = yf1; yf_s2 = yf2;
     }
-    __syncthreads();
-    // transposed store of the y-line right-hand sides: column x, rows [ys, ye) at offset y-2
+    // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
     if (out_lane) {
         float *dst = p.DyT + (size_t)x * PT + (ys - 2);
         const int nr = ye - ys;
@@ -435,49 +438,6 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 
 // ------------------------------------------------------------------ line solves
 
-// In-place cyclic reduction of one line held in LDS (a, b, c, d of n unknowns).
-// Level k keeps unknowns at positions 2^k (m+1) - 1; their neighbours at level k-1
-// sit 2^(k-1) away; a neighbour at or beyond n is the reference's (0, 1, 0 | 0) margin.
-// Back-substitution leaves the solution in d.  Arithmetic as in engine.cpp:2719-2735
-// and compute.hlsl:849-852, 913.
-template <int NT>
-__device__ void cr_solve_lds(int n, int L, float *a, float *b, float *c, float *d)
-{
-    const int tid = threadIdx.x;
-    for (int k = 1; k < L; ++k) {
-        const int s = 1 << (k - 1);
-        const int cnt = n >> k;
-        for (int m = tid; m < cnt; m += NT) {
-            const int pp = ((m + 1) << k) - 1;
-            const int pl = pp - s, pr = pp + s;
-            const float aL = a[pl], bL = b[pl], cL = c[pl], dL = d[pl];
-            float aR = 0.f, bR = 1.f, cR = 0.f, dR = 0.f;
-            if (pr < n) { aR = a[pr]; bR = b[pr]; cR = c[pr]; dR = d[pr]; }
-            const float ap = a[pp], bp = b[pp], cp = c[pp], dp = d[pp];
-            const float k1 = ap / bL;
-            const float k2 = cp / bR;
-            a[pp] = -aL * k1;
-            b[pp] = bp - cL * k1 - aR * k2;
-            c[pp] = -cR * k2;
-            d[pp] = dp - dL * k1 - dR * k2;
-        }
-        __syncthreads();
-    }
-    for (int k = L - 1; k >= 0; --k) {
-        const int s = 1 << k;
-        const int nk = n >> k;
-        const int cnt = (nk + 1) >> 1;
-        for (int q = tid; q < cnt; q += NT) {
-            const int pp = s * (2 * q + 1) - 1;
-            const float xl = (pp - s >= 0) ? d[pp - s] : 0.f;
-            const float xr = (pp + s < n) ? d[pp + s] : 0.f;
-            d[pp] = (d[pp] - a[pp] * xl - c[pp] * xr) / b[pp];
-        }
-        __syncthreads();
-    }
-}
-
-constexpr int kSolveThreads = 256;
 
 // Irregular waves: one wave per (side, strip index, line cell) computes that cell's
 // pairwise-tree sum of the component texels (ColumnSumReduce/RowSumReduce order).
@@ -537,14 +497,155 @@ __device__ void irregular_line(const IrregularSide &S, int strip, int line, floa
 }
 
 
-__global__ __launch_bounds__(kSolveThreads) void solve_lines(const LineParams p, const IrregularAll irr)
+// One wave per line, each lane holding a contiguous chunk of C unknowns in registers.
+// The eliminations are those of the reference's cyclic reduction (reduced equation m of
+// level k built from unknowns 2m, 2m+1, 2m+2 of level k-1; back-substitution level by
+// level), so the arithmetic, and therefore every rounding, is unchanged; only the
+// placement differs.  Level k keeps the unknowns at positions 2^k (m+1) - 1: for
+// 2^k <= C they lie inside one lane's chunk, above that at the chunk's last slot of
+// lanes 2^(k-log2 C) (m+1) - 1.  Neighbours outside the chunk come from other lanes
+// through cross-lane reads; positions >= n are the reference's (0, 1, 0 | 0) margin.
+struct Eq { float a, b, c, d; };
+
+__device__ __forceinline__ Eq shfl_eq(const Eq &e, int src)
 {
-    extern __shared__ float lds[];
+    return Eq{__shfl(e.a, src, 64), __shfl(e.b, src, 64), __shfl(e.c, src, 64), __shfl(e.d, src, 64)};
+}
+
+__device__ __forceinline__ Eq reduce_eq(const Eq &L, const Eq &H, const Eq &R)
+{
+    const float k1 = H.a / L.b;
+    const float k2 = H.c / R.b;
+    Eq o;
+    o.a = -L.a * k1;
+    o.b = H.b - L.c * k1 - R.a * k2;
+    o.c = -R.c * k2;
+    o.d = H.d - L.d * k1 - R.d * k2;
+    return o;
+}
+
+// LDS staging index with one pad word per 32: lane chunks of C >= 32 words start in
+// different banks.
+__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
+
+template <int C, int W>
+struct LineSmem {
+    float D[64 * W * C + 2 * W * C + 8];      // padded right-hand side / solution staging
+    float B[64 * W * C + 2 * W * C + 8];      // padded depths max(0, sl - BA)
+    Eq top[64 * W];                            // one unknown per thread above the chunk levels
+    Eq xch[2][64 * W];                         // chunk-level neighbour exchange (double-buffered)
+};
+
+// Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
+template <int C, int W>
+__device__ void cr_line_block(int n, const float *D, const float *BAl, float *out, float kcoef, float del2, float cs,
+                              LineSmem<C, W> &sm)
+{
+    constexpr int NT = 64 * W;
+    constexpr int K0 = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : C == 16 ? 4 : 5;
+    constexpr int KT = W == 1 ? 6 : W == 2 ? 7 : W == 4 ? 8 : 9;     // log2(NT)
+    const int t = threadIdx.x;
+    const int base = t * C;
+    // coalesced loads of the right-hand side and the bottom line into LDS
+    for (int i = t; i < n; i += NT) sm.D[pad32(i)] = D[i];
+    for (int i = t; i < n + 4; i += NT) sm.B[pad32(i)] = fmaxr(0.f, cs - BAl[i]);
+    __syncthreads();
+    Eq q[C];
+    // level-0 coefficients from the depths at offsets -2..+2 (engine.cpp:2656-2680)
+    {
+        float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+        if (base < n + 4) e0 = sm.B[pad32(base)];
+        if (base + 1 < n + 4) e1 = sm.B[pad32(base + 1)];
+        if (base + 2 < n + 4) e2 = sm.B[pad32(base + 2)];
+        if (base + 3 < n + 4) e3 = sm.B[pad32(base + 3)];
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const int el = base + i;
+            const float e4 = el + 4 < n + 4 ? sm.B[pad32(el + 4)] : 0.f;
+            if (el < n) {
+                line_coef(kcoef, del2, e0, e1, e2, e3, e4, el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
+                q[i].d = sm.D[pad32(el)];
+            } else {
+                q[i] = Eq{0.f, 1.f, 0.f, 0.f};
+            }
+            e0 = e1; e1 = e2; e2 = e3; e3 = e4;
+        }
+    }
+    const Eq margin{0.f, 1.f, 0.f, 0.f};
+    // ---- reduction inside the chunk: levels 1..K0
+#pragma unroll
+    for (int k = 1; k <= K0; ++k) {
+        const int s = 1 << (k - 1);
+        // the chunk's last level-k unknown needs the next thread's level-(k-1) unknown at s-1
+        sm.xch[k & 1][t] = q[s - 1];
+        __syncthreads();
+        Eq nr = t + 1 < NT ? sm.xch[k & 1][t + 1] : margin;
+        if (base + C - 1 + s >= n) nr = margin;
+#pragma unroll
+        for (int j = 0; j < (C >> k); ++j) {
+            const int pp = ((j + 1) << k) - 1;
+            if (base + pp >= n) continue;
+            const Eq &L = q[pp - s];
+            const Eq R = (pp + s < C) ? ((base + pp + s < n) ? q[pp + s] : margin) : nr;
+            q[pp] = reduce_eq(L, q[pp], R);
+        }
+    }
+    // ---- reduction across threads: levels K0+1 .. K0+KT, in LDS
+    const int mypos = base + C - 1;
+    sm.top[t] = q[C - 1];
+    __syncthreads();
+#pragma unroll
+    for (int j = 1; j <= KT; ++j) {
+        const int s = 1 << (j - 1);
+        if (((t + 1) & ((1 << j) - 1)) == 0 && mypos < n) {
+            const Eq L = sm.top[t - s];
+            const Eq R = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top[t + s] : margin;
+            sm.top[t] = reduce_eq(L, sm.top[t], R);
+        }
+        __syncthreads();
+    }
+    // ---- back-substitution across threads: levels K0+KT .. K0 (solution in .d)
+#pragma unroll
+    for (int j = KT; j >= 0; --j) {
+        const int s = 1 << j;
+        if (((t + 1) & ((2 << j) - 1)) == s && mypos < n) {
+            const float xl = t - s >= 0 ? sm.top[t - s].d : 0.f;
+            const float xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top[t + s].d : 0.f;
+            Eq e = sm.top[t];
+            sm.top[t].d = (e.d - e.a * xl - e.c * xr) / e.b;
+        }
+        __syncthreads();
+    }
+    q[C - 1].d = sm.top[t].d;
+    const float prev_last = t > 0 ? sm.top[t - 1].d : 0.f;
+    // ---- back-substitution inside the chunk: levels K0-1 .. 0
+#pragma unroll
+    for (int k = K0 - 1; k >= 0; --k) {
+        const int s = 1 << k;
+#pragma unroll
+        for (int qq = 0; qq < (C >> (k + 1)); ++qq) {
+            const int pp = s * (2 * qq + 1) - 1;
+            if (base + pp >= n) continue;
+            const float xl = (pp - s >= 0) ? q[pp - s].d : prev_last;
+            const float xr = (base + pp + s < n) ? q[pp + s].d : 0.f;
+            q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+        if (base + i < n) sm.D[pad32(base + i)] = q[i].d;
+    __syncthreads();
+    for (int i = t; i < n; i += NT) out[i] = sm.D[pad32(i)];
+}
+
+template <int C, int W>
+__global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, const IrregularAll irr)
+{
     const int bid = blockIdx.x;
     const int nlines = p.nlines_x + p.nlines_y;
     if (bid >= nlines) {
-        // irregular-wave lines: 4 per block (one per wave)
-        int idx = (bid - nlines) * (kSolveThreads / 64) + (threadIdx.x >> 6);
+        // irregular-wave lines: W per block (one per wave)
+        int idx = (bid - nlines) * W + (threadIdx.x >> 6);
         for (int s = 0; s < 4; ++s) {
             if (idx < irr.count[s]) {
                 const IrregularSide &S = irr.side[s];
@@ -555,26 +656,15 @@ __global__ __launch_bounds__(kSolveThreads) void solve_lines(const LineParams p,
         }
         return;
     }
+    __shared__ LineSmem<C, W> sm;
     const bool xdir = bid < p.nlines_x;
-    const int n = xdir ? p.nx : p.ny;
-    const int L = xdir ? p.Lx : p.Ly;
-    const float del2 = xdir ? p.dx2 : p.dy2;
-    const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;   // row y or column x
-    const float *D = xdir ? p.Dx + (size_t)line * p.P + 2 : p.DyT + (size_t)line * p.PT;
-    const float *BAl = xdir ? p.BA + (size_t)line * p.P : p.BAT + (size_t)line * p.PT;
-    float *a = lds, *b = lds + n, *c = lds + 2 * n, *d = lds + 3 * n;
-    const float cs = p.coef_sl;
-    for (int i = threadIdx.x; i < n; i += kSolveThreads) {
-        const float e0 = fmaxr(0.f, cs - BAl[i]), e1 = fmaxr(0.f, cs - BAl[i + 1]), e2 = fmaxr(0.f, cs - BAl[i + 2]);
-        const float e3 = fmaxr(0.f, cs - BAl[i + 3]), e4 = fmaxr(0.f, cs - BAl[i + 4]);
-        float A, B, C;
-        line_coef(p.kcoef, del2, e0, e1, e2, e3, e4, i == 0, i == n - 1, A, B, C);
-        a[i] = A; b[i] = B; c[i] = C; d[i] = D[i];
-    }
-    __syncthreads();
-    cr_solve_lds<kSolveThreads>(n, L, a, b, c, d);
-    float *out = xdir ? p.hu_out + (size_t)line * p.P + 2 : p.hvT_out + (size_t)line * p.PT + 2;
-    for (int i = threadIdx.x; i < n; i += kSolveThreads) out[i] = d[i];
+    const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
+    if (xdir)
+        cr_line_block<C, W>(p.nx, p.Dx + (size_t)line * p.P, p.BA + (size_t)line * p.P,
+                            p.hu_out + (size_t)line * p.P + 2, p.kcoef, p.dx2, p.coef_sl, sm);
+    else
+        cr_line_block<C, W>(p.ny, p.DyT + (size_t)line * p.PT, p.BAT + (size_t)line * p.PT,
+                            p.hvT_out + (size_t)line * p.PT + 2, p.kcoef, p.dy2, p.coef_sl, sm);
 }
 
 // ------------------------------------------------------------------ boundaries
@@ -763,7 +853,7 @@ namespace cel {
 
 hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st)
 {
-    const dim3 grid(p.nstrips * p.nbands), block(64);
+    const dim3 grid(8 * ((p.nstrips * p.nbands + 7) / 8)), block(64);
     switch (scheme) {
     case 0: hipLaunchKernelGGL(explicit_step<0>, grid, block, 0, st, p); break;
     case 1: hipLaunchKernelGGL(explicit_step<1>, grid, block, 0, st, p); break;
@@ -775,20 +865,28 @@ hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st)
 
 hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_t st)
 {
-    static bool attr_set = false;
-    const size_t lds = sizeof(float) * 4 * (size_t)(p.nx > p.ny ? p.nx : p.ny);
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(solve_lines),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 4 * 4 * CEL_MAX_LINE_);
-        if (e != hipSuccess) return e;
-        attr_set = true;
+    const int nmax = p.nx > p.ny ? p.nx : p.ny;
+    if (nmax <= 64 * 8 * 16) {
+        int irr_lines = 0;
+        for (int s = 0; s < 4; ++s) irr_lines += irr.count[s];
+        // lines of up to 64 W C unknowns: C per thread in registers, W waves per line
+#define CEL_SOLVE(C, W)                                                                              \
+    do {                                                                                            \
+        const dim3 grid(p.nlines_x + p.nlines_y + (irr_lines + (W) - 1) / (W)), block(64 * (W));    \
+        hipLaunchKernelGGL((solve_lines_block<C, W>), grid, block, 0, st, p, irr);                  \
+    } while (0)
+        if (nmax <= 64) CEL_SOLVE(1, 1);
+        else if (nmax <= 128) CEL_SOLVE(2, 1);
+        else if (nmax <= 256) CEL_SOLVE(4, 1);
+        else if (nmax <= 512) CEL_SOLVE(8, 1);
+        else if (nmax <= 1024) CEL_SOLVE(8, 2);
+        else if (nmax <= 2048) CEL_SOLVE(8, 4);
+        else if (nmax <= 4096) CEL_SOLVE(16, 4);
+        else CEL_SOLVE(16, 8);
+#undef CEL_SOLVE
+        return hipGetLastError();
     }
-    int irr_lines = 0;
-    for (int s = 0; s < 4; ++s) irr_lines += irr.count[s];
-    const int irr_blocks = (irr_lines + (kSolveThreads / 64) - 1) / (kSolveThreads / 64);
-    const dim3 grid(p.nlines_x + p.nlines_y + irr_blocks), block(kSolveThreads);
-    hipLaunchKernelGGL(solve_lines, grid, block, lds, st, p, irr);
-    return hipGetLastError();
+    return hipErrorInvalidValue;   // lines longer than CEL_MAX_LINE are rejected at creation
 }
 
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)

@@ -10,7 +10,10 @@
 
 namespace cel {
 
-constexpr int kBandRows = 16;   // rows one wave marches through in the explicit kernel
+#ifndef CEL_BAND_ROWS
+#define CEL_BAND_ROWS 8
+#endif
+constexpr int kBandRows = CEL_BAND_ROWS;   // rows one wave marches through in the explicit kernel
 constexpr int kStripCols = 60;  // interior columns one wave produces (64 lanes - 2 halo each side)
 
 struct ExplicitParams {
@@ -26,7 +29,7 @@ struct ExplicitParams {
     const float *Gp[3], *FGp[2], *Cw, *Chu, *ChvT;
     // outputs
     float *w_new;    // w^{n+1} (interior)
-    float *Dx;       // right-hand side of the x-line systems (row-major)
+    float *Dx;       // right-hand side of the x-line systems, [y * P + (x - 2)]
     float *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
     float *Gs[3], *FGs[2];
     // SimConstBuffer (compute.hlsl:65-94)

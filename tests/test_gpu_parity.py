@@ -139,3 +139,18 @@ def test_plain_beach_1000_steps():
     a, b = run_pair(sc, 1000)
     err = check(sc, a, b)
     print("plain_beach 1000 steps relative errors", err)
+
+
+def test_long_lines_register_chunks():
+    """x-lines of 2048 and 1500 cells (32 and 24 unknowns per lane) and 1100-cell y-lines."""
+    for nx, ny in ((2048, 20), (1500, 24), (24, 1100)):
+        sc = S.small(nx=nx, ny=ny, seed=13)
+        a, b = run_pair(sc, 40)
+        check(sc, a, b, exact=True)
+
+
+def test_lines_longer_than_a_wave_chunk():
+    """2600-cell x-lines: 16 unknowns per thread, four waves per line."""
+    sc = S.small(nx=2600, ny=12, seed=14)
+    a, b = run_pair(sc, 30)
+    check(sc, a, b, exact=True)
