@@ -26,18 +26,32 @@ namespace cel {
 
 // ------------------------------------------------------------------ helpers
 
-__device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
-__device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b; }
+// IEEE min/max (v_min_f32 / v_max_f32, fused into v_min3/v_max3).  They agree with the
+// shader's max/min on every non-NaN input up to the sign of a zero result, which never
+// changes a nonzero value downstream.
+__device__ __forceinline__ float fmaxr(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float fminr(float a, float b) { return fminf(a, b); }
 
-// lane i receives lane i+1 / i-1 (edge lanes get their own value; they are halo lanes)
-__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
+// lane i receives lane i+1 / i-1 through a DPP wave shift (edge lanes keep their own
+// value; they are halo lanes)
+__device__ __forceinline__ float from_right(float v)
+{
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));   // wave_shl:1
+}
+__device__ __forceinline__ float from_left(float v)
+{
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));   // wave_shr:1
+}
 
-// MinMod, compute.hlsl:292-296
+// MinMod, compute.hlsl:292-296: the minimum of three positive slopes, the maximum of three
+// negative ones, else zero -- selected through min3/max3.
 __device__ __forceinline__ float minmod3(float a, float b, float c)
 {
-    return (a > 0.f && b > 0.f && c > 0.f) ? fminr(fminr(a, b), c)
-         : (a < 0.f && b < 0.f && c < 0.f) ? fmaxr(fmaxr(a, b), c) : 0.f;
+    const float mn = fminf(fminf(a, b), c);
+    const float mx = fmaxf(fmaxf(a, b), c);
+    return mn > 0.f ? mn : (mx < 0.f ? mx : 0.f);
 }
 
 // Reconstruct, compute.hlsl:298-315: half-slope of the generalized minmod limiter
@@ -49,16 +63,17 @@ __device__ __forceinline__ float half_slope(float two_theta, float west, float h
     return 0.25f * minmod3(z1, z2, z3);
 }
 
-// CorrectW, compute.hlsl:317-328
+// CorrectW, compute.hlsl:317-328, as selects
 __device__ __forceinline__ void correct_w(float B_west, float B_east, float w_bar, float &w_west, float &w_east)
 {
-    if (w_east < B_east) {
-        w_east = B_east;
-        w_west = fmaxr(B_west, 2.f * w_bar - B_east);
-    } else if (w_west < B_west) {
-        w_east = fmaxr(B_east, 2.f * w_bar - B_west);
-        w_west = B_west;
-    }
+    const bool c1 = w_east < B_east;
+    const bool c2 = !c1 && w_west < B_west;
+    const float alt_w = fmaxr(B_west, 2.f * w_bar - B_east);
+    const float alt_e = fmaxr(B_east, 2.f * w_bar - B_west);
+    const float ne = c1 ? B_east : (c2 ? alt_e : w_east);
+    const float nw = c1 ? alt_w : (c2 ? B_west : w_west);
+    w_east = ne;
+    w_west = nw;
 }
 
 // desingularised 1/h of CalcUV, compute.hlsl:330-340
@@ -72,8 +87,20 @@ __device__ __forceinline__ float divide_by_h(float h, float eps)
 // NumericalFlux, compute.hlsl:363-370
 __device__ __forceinline__ float num_flux(float ap, float am, float Fp, float Fm, float dU)
 {
-    if (ap - am != 0.f) return (ap * Fm - am * Fp + ap * am * dU) / (ap - am);
-    return 0.f;
+    const float den = ap - am;
+    const float q = (ap * Fm - am * Fp + ap * am * dU) / den;   // discarded when den == 0
+    return den != 0.f ? q : 0.f;
+}
+
+// Element access through a 32-bit byte offset from a uniform base: maps to the
+// scalar-base + vector-offset global addressing mode (planes are < 4 GiB).
+__device__ __forceinline__ float ld32(const float *base, unsigned idx)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (idx << 2));
+}
+__device__ __forceinline__ void st32(float *base, unsigned idx, float v)
+{
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (idx << 2)) = v;
 }
 
 struct Face { float h, u, v; };
@@ -203,7 +230,9 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     // register windows over rows; index 0 = y-2 ... 4 = y+2
     float w_[5], BA_[5], hu_[4], hv_[4], BY_[4];
     // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
-    auto ldrow = [&](const float *base, int row) { return base[(size_t)row * P + xc]; };
+    // 32-bit element offsets: planes are far below 4 G elements, and a uniform base with a
+    // 32-bit offset maps to the scalar-base addressing mode (no 64-bit address arithmetic)
+    auto ldrow = [&](const float *base, int row) { return ld32(base, (unsigned)row * (unsigned)P + (unsigned)xc); };
 
     // prologue: rows ys-2 .. ys+1
     for (int j = 0; j < 4; ++j) {
@@ -277,7 +306,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
         const float BYc = BY_[1];
 
         if (out_lane) {
-            const size_t i = (size_t)y * P + x;
+            const unsigned i = (unsigned)y * (unsigned)P + (unsigned)x;
             const float h = fmaxr(0.f, wc - BAc);
             const float d_here = fmaxr(0.f, sl - BAc);
             const float d2_here = d_here * d_here;
@@ -365,10 +394,10 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
                 huOut = Ax * hul + Bx * huc + Cx * hur + dt * dg;
                 hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt * db;
             } else {
-                const float o0 = p.Go[0][i], o1 = p.Go[1][i], o2 = p.Go[2][i];
-                const float oo0 = p.Goo[0][i], oo1 = p.Goo[1][i], oo2 = p.Goo[2][i];
-                const float Fo = p.FGo[0][i], Go_ = p.FGo[1][i];
-                const float Foo = p.FGoo[0][i], Goo_ = p.FGoo[1][i];
+                const float o0 = ld32(p.Go[0], i), o1 = ld32(p.Go[1], i), o2 = ld32(p.Go[2], i);
+                const float oo0 = ld32(p.Goo[0], i), oo1 = ld32(p.Goo[1], i), oo2 = ld32(p.Goo[2], i);
+                const float Fo = ld32(p.FGo[0], i), Go_ = ld32(p.FGo[1], i);
+                const float Foo = ld32(p.FGoo[0], i), Goo_ = ld32(p.FGoo[1], i);
                 if (SCHEME == 1) {
                     wOut = wc + dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
                     huOut = Ax * hul + Bx * huc + Cx * hur + dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1) +
@@ -390,13 +419,13 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
                     hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2) +
                             dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
                 } else {
-                    const float cw = p.Cw[i];
-                    const float chu = p.Chu[i], chul = p.Chu[i - 1], chur = p.Chu[i + 1];
+                    const float cw = ld32(p.Cw, i);
+                    const float chu = ld32(p.Chu, i), chul = ld32(p.Chu, i - 1u), chur = ld32(p.Chu, i + 1u);
                     const float chv = chvblk[y - (ys - 2)][lane];
                     const float chvd = chvblk[y - 1 - (ys - 2)][lane];
                     const float chvu = chvblk[y + 1 - (ys - 2)][lane];
-                    const float pr0 = p.Gp[0][i], pr1 = p.Gp[1][i], pr2 = p.Gp[2][i];
-                    const float Fp = p.FGp[0][i], Gp_ = p.FGp[1][i];
+                    const float pr0 = ld32(p.Gp[0], i), pr1 = ld32(p.Gp[1], i), pr2 = ld32(p.Gp[2], i);
+                    const float Fp = ld32(p.FGp[0], i), Gp_ = ld32(p.FGp[1], i);
                     wOut = cw + dt / 24.f * (9.f * dr + 19.f * pr0 - 5.f * o0 + 1.f * oo0);
                     huOut = Ax * chul + Bx * chu + Cx * chur + dt / 24.f * (9.f * dg + 19.f * pr1 - 5.f * o1 + 1.f * oo1) +
                             F_star - Fp;
@@ -404,14 +433,14 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
                             G_star - Gp_;
                 }
             }
-            p.w_new[i] = wOut;
-            p.Dx[(size_t)y * P + (x - 2)] = huOut;
+            st32(p.w_new, i, wOut);
+            st32(p.Dx, i - 2u, huOut);
             dystg[y - ys][lane] = hvOut;
-            p.Gs[0][i] = dr;
-            p.Gs[1][i] = dg;
-            p.Gs[2][i] = db;
-            p.FGs[0][i] = F_star;
-            p.FGs[1][i] = G_star;
+            st32(p.Gs[0], i, dr);
+            st32(p.Gs[1], i, dg);
+            st32(p.Gs[2], i, db);
+            st32(p.FGs[0], i, F_star);
+            st32(p.FGs[1], i, G_star);
         }
         // ---- shift windows
         w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];
