@@ -94,8 +94,8 @@ void ShallowWaterEngine::allocate()
 {
     const size_t plane = (size_t)P_ * Y_ * sizeof(float);
     const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
-    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2;   // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu
-    const int ncol = 2 + 1 + 1 + 1;                       // hvT x2; BAT; DyT; ChvT
+    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6;   // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; static
+    const int ncol = 2 + 1 + 1 + 2;                           // hvT x2; DyT; ChvT; KyT, QyT
     arena_bytes_ = plane * nrow + tplane * ncol;
     void *mem = nullptr;
     check(hipMalloc(&mem, arena_bytes_), "hipMalloc(state)");
@@ -111,7 +111,8 @@ void ShallowWaterEngine::allocate()
     for (auto &g : FG_) for (auto &c : g) c = rp();
     for (auto &c : FGp_) c = rp();
     Cw_ = rp(); Chu_ = rp();
-    hvT_[0] = cp(); hvT_[1] = cp(); BAT_ = cp(); DyT_ = cp(); ChvT_ = cp();
+    Kx_ = rp(); Qx_ = rp(); Ky_ = rp(); Qy_ = rp(); DDx_ = rp(); DDy_ = rp();
+    hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); KyT_ = cp(); QyT_ = cp();
 }
 
 void ShallowWaterEngine::uploadBottom(const float *bottom3)
@@ -122,11 +123,9 @@ void ShallowWaterEngine::uploadBottom(const float *bottom3)
         check(hipMemcpy2DAsync(dst[c], (size_t)P_ * 4, bottom3 + c * N, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
                                hipMemcpyHostToDevice, stream_), "upload bottom");
     hostBA_.assign(bottom3 + 2 * N, bottom3 + 3 * N);
-    std::vector<float> t((size_t)X_ * Y_);
-    for (int y = 0; y < Y_; ++y)
-        for (int x = 0; x < X_; ++x) t[(size_t)x * Y_ + y] = hostBA_[(size_t)y * X_ + x];
-    check(hipMemcpy2DAsync(BAT_, (size_t)PT_ * 4, t.data(), (size_t)Y_ * 4, (size_t)Y_ * 4, X_,
-                           hipMemcpyHostToDevice, stream_), "upload BA^T");
+    StaticParams sp;
+    fillStatic(sp);
+    check(launch_static_fields(sp, stream_), "static_fields");
     check(hipStreamSynchronize(stream_), "upload sync");
 }
 
@@ -478,10 +477,24 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     p.der[6] = -(d2) / (d1 * (d1 + d2));
     p.der[7] = (d1 + d2) / (d1 * d2);
     p.der[8] = -(d1 + 2 * d2) / (d2 * (d1 + d2));
+    p.Kx = Kx_; p.Qx = Qx_; p.Ky = Ky_; p.Qy = Qy_; p.DDx = DDx_; p.DDy = DDy_;
+}
+
+// createTridiagonalCoefTextures' level-0 inputs (engine.cpp:2619-2625) and the depth
+// gradients of Pass3 (compute.hlsl:641-697)
+void ShallowWaterEngine::fillStatic(StaticParams &p) const
+{
+    std::memset(&p, 0, sizeof(p));
+    p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
+    p.BA = BA_;
+    p.Kx = Kx_; p.Qx = Qx_; p.Ky = Ky_; p.Qy = Qy_; p.DDx = DDx_; p.DDy = DDy_; p.KyT = KyT_; p.QyT = QyT_;
     p.kcoef = 1 / 15.0f + 1 / 3.0f;
     p.dx2 = dx_ * dx_;
     p.dy2 = dy_ * dy_;
     p.coef_sl = s_.is_boussinesq ? s_.still_water_elevation : -9999.f;
+    p.sea_level = s_.still_water_elevation;
+    p.one_over_dx = 1.0f / dx_;
+    p.one_over_dy = 1.0f / dy_;
 }
 
 void ShallowWaterEngine::fillLines(LineParams &p) const
@@ -489,12 +502,10 @@ void ShallowWaterEngine::fillLines(LineParams &p) const
     std::memset(&p, 0, sizeof(p));
     p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
     p.nlines_x = ny_; p.nlines_y = nx_;
-    p.Dx = Dx_; p.BA = BA_; p.DyT = DyT_; p.BAT = BAT_;
+    p.Dx = Dx_; p.Kx = Kx_; p.Qx = Qx_;
+    p.DyT = DyT_; p.KyT = KyT_; p.QyT = QyT_;
     const int n = 1 - cur_;
     p.hu_out = hu_[n]; p.hvT_out = hvT_[n];
-    p.kcoef = 1 / 15.0f + 1 / 3.0f;
-    p.dx2 = dx_ * dx_; p.dy2 = dy_ * dy_;
-    p.coef_sl = s_.is_boussinesq ? s_.still_water_elevation : -9999.f;
     p.Lx = Lx_; p.Ly = Ly_;
 }
 
@@ -672,12 +683,12 @@ bool ShallowWaterEngine::kernelTime(const std::string &name, float *ms, int *lau
 
 void ShallowWaterEngine::coefficients(float *ax, float *ay)
 {
-    LineParams lp;
-    fillLines(lp);
+    StaticParams sp;
+    fillStatic(sp);
     const size_t N = (size_t)X_ * Y_;
     float *d = nullptr;
     check(hipMalloc(&d, N * 6 * 4), "hipMalloc");
-    check(launch_export_coefficients(lp, BA_, BAT_, d, d + 3 * N, stream_), "export_coefficients");
+    check(launch_export_coefficients(sp, d, d + 3 * N, stream_), "export_coefficients");
     check(hipMemcpyAsync(ax, d, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
     check(hipMemcpyAsync(ay, d + 3 * N, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
     check(hipStreamSynchronize(stream_), "sync");

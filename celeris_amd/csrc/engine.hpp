@@ -26,8 +26,8 @@ struct Error : std::runtime_error {
 hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st);
 hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_t st);
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
-hipError_t launch_export_coefficients(const LineParams &p, const float *BA, const float *BAT, float *ax, float *ay,
-                                      hipStream_t st);
+hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
+hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st);
 
 // st_getCountOfMatrices, engine.cpp:741-748
 int count_of_matrices(int n);
@@ -77,6 +77,7 @@ private:
     void validateBoundaries() const;
     void fillExplicit(ExplicitParams &p) const;
     void fillLines(LineParams &p) const;
+    void fillStatic(StaticParams &p) const;
     void check(hipError_t e, const char *what) const;
     void timeSeries(int side, float t, float &eta, float &hu, float &hv);
 
@@ -91,7 +92,8 @@ private:
     char *arena_ = nullptr;
     size_t arena_bytes_ = 0, table_bytes_ = 0;
     float *w_[2], *hu_[2], *hvT_[2];
-    float *BY_, *BX_, *BA_, *BAT_;
+    float *BY_, *BX_, *BA_;
+    float *Kx_, *Qx_, *Ky_, *Qy_, *DDx_, *DDy_, *KyT_, *QyT_;   // static per-cell fields
     float *Dx_, *DyT_;
     float *G_[3][3];          // ring slot, component
     float *FG_[3][2];

@@ -157,20 +157,47 @@ __device__ __forceinline__ void flux_y(float g, float half_g, const Face &N, con
                   Sn.h * Sn.v - N.h * N.v);
 }
 
-// Level-0 tridiagonal coefficients (engine.cpp:2664-2680): A, B, C of a line cell
-// from the depths at offsets -2..+2 along the line.
-__device__ __forceinline__ void line_coef(float kcoef, float d2, float dm2, float dm1, float d0, float dp1, float dp2,
-                                          bool first, bool last, float &A, float &B, float &C)
+// Level-0 tridiagonal coefficients (engine.cpp:2664-2680) of a line cell: K1 and K2/72
+// from the depths at offsets -2..+2 along the line, then A, B, C.
+__device__ __forceinline__ void line_kq(float kcoef, float d2, float dm2, float dm1, float d0, float dp1, float dp2,
+                                        float &K1, float &Q)
 {
-    const float K1 = kcoef * (d0 * d0 / d2);
+    K1 = kcoef * (d0 * d0 / d2);
     const float K2 = (d0 / d2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
-    const float K2_72 = K2 / 72.f;
-    A = first ? 0.f : -K1 + K2_72;
+    Q = K2 / 72.f;
+}
+__device__ __forceinline__ void line_abc(float K1, float Q, bool first, bool last, float &A, float &B, float &C)
+{
+    A = first ? 0.f : -K1 + Q;
     B = 1.f + 2.f * K1;
-    C = last ? 0.f : -(K1 + K2_72);
+    C = last ? 0.f : -(K1 + Q);
 }
 
 // ------------------------------------------------------------------ explicit step
+
+// Per-cell inputs of Pass3 that are read once: Adams-Bashforth history and static fields.
+struct CellIn {
+    float o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
+    float Kx, Qx, Ky, Qy, ddx, ddy;
+};
+
+template <int SCHEME>
+__device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
+{
+    CellIn c;
+    if (SCHEME != 0) {
+        c.o0 = ld32(p.Go[0], i); c.o1 = ld32(p.Go[1], i); c.o2 = ld32(p.Go[2], i);
+        c.oo0 = ld32(p.Goo[0], i); c.oo1 = ld32(p.Goo[1], i); c.oo2 = ld32(p.Goo[2], i);
+        c.Fo = ld32(p.FGo[0], i); c.Go = ld32(p.FGo[1], i);
+        c.Foo = ld32(p.FGoo[0], i); c.Goo = ld32(p.FGoo[1], i);
+    } else {
+        c.o0 = c.o1 = c.o2 = c.oo0 = c.oo1 = c.oo2 = c.Fo = c.Go = c.Foo = c.Goo = 0.f;
+    }
+    c.Kx = ld32(p.Kx, i - 2u); c.Qx = ld32(p.Qx, i - 2u);
+    c.Ky = ld32(p.Ky, i - 2u); c.Qy = ld32(p.Qy, i - 2u);
+    c.ddx = ld32(p.DDx, i - 2u); c.ddy = ld32(p.DDy, i - 2u);
+    return c;
+}
 
 // SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
 template <int SCHEME>
@@ -228,7 +255,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     const float sl = p.sea_level;
 
     // register windows over rows; index 0 = y-2 ... 4 = y+2
-    float w_[5], BA_[5], hu_[4], hv_[4], BY_[4];
+    float w_[5], hu_[4], hv_[4], BY_[4];
     // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
     // 32-bit element offsets: planes are far below 4 G elements, and a uniform base with a
     // 32-bit offset maps to the scalar-base addressing mode (no 64-bit address arithmetic)
@@ -238,7 +265,6 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     for (int j = 0; j < 4; ++j) {
         const int row = ys - 2 + j;
         w_[j] = ldrow(p.w, row);
-        BA_[j] = ldrow(p.BA, row);
     }
     for (int j = 0; j < 3; ++j) {
         const int row = ys - 1 + j;
@@ -263,14 +289,30 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     float yf_s0, yf_s1, yf_s2;   // yflux at row y-1
     flux_y(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
 
+    // software pipelining: the loads a row needs are issued one row ahead
+    const int ylast = p.ny + 3;
+    float nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
+    float nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys);
+    CellIn nx_c = load_cell<SCHEME>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
+
     for (int y = ys; y < ye; ++y) {
-        // ---- load row y+2 (w, BA) and row y+2 for hu/hv/BY into slot 3
-        w_[4] = ldrow(p.w, y + 2);
-        BA_[4] = ldrow(p.BA, y + 2);
-        hu_[3] = ldrow(p.hu, y + 2);
+        // ---- rows y+2 (w, hu, hv, BY) and y (BX, BA, per-cell inputs), prefetched
+        w_[4] = nx_w;
+        hu_[3] = nx_hu;
         hv_[3] = hvblk[y + 2 - (ys - 2)][lane];
-        BY_[3] = ldrow(p.BY, y + 2);
-        const float BXc = ldrow(p.BX, y);
+        BY_[3] = nx_BY;
+        const float BXc = nx_BX;
+        const float BAc = nx_BA;
+        const CellIn cin = nx_c;
+        {
+            const int yn = min(y + 3, ylast), yc = min(y + 1, ye - 1);
+            nx_w = ldrow(p.w, yn);
+            nx_hu = ldrow(p.hu, yn);
+            nx_BY = ldrow(p.BY, yn);
+            nx_BX = ldrow(p.BX, yc);
+            nx_BA = ldrow(p.BA, yc);
+            nx_c = load_cell<SCHEME>(p, (unsigned)yc * (unsigned)P + (unsigned)xc);
+        }
 
         // ---- faces of row y+1 along y: S face used now, N face kept
         Face S1, N1;
@@ -299,9 +341,6 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
         const float wll = from_left(wl), wrr = from_right(wr);
         const float wul = from_left(w_[3]), wur = from_right(w_[3]);
         const float wdl = from_left(w_[1]), wdr = from_right(w_[1]);
-        const float BAc = BA_[2];
-        const float BAl = from_left(BAc), BAr = from_right(BAc);
-        const float BAll = from_left(BAl), BArr = from_right(BAr);
         const float BYs = BY_[0];
         const float BYc = BY_[1];
 
@@ -311,14 +350,6 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
             const float d_here = fmaxr(0.f, sl - BAc);
             const float d2_here = d_here * d_here;
             const float d3_here = d2_here * d_here;
-            const float d_left = fmaxr(0.f, sl - BAl);
-            const float d_right = fmaxr(0.f, sl - BAr);
-            const float d_down = fmaxr(0.f, sl - BA_[1]);
-            const float d_up = fmaxr(0.f, sl - BA_[3]);
-            const float d_left_left = fmaxr(0.f, sl - BAll);
-            const float d_right_right = fmaxr(0.f, sl - BArr);
-            const float d_down_down = fmaxr(0.f, sl - BA_[0]);
-            const float d_up_up = fmaxr(0.f, sl - BA_[4]);
 
             const float eta_here = wc - sl;
             const float eta_left = wl - sl, eta_right = wr - sl;
@@ -334,8 +365,8 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
             const float one_over_d2y = ody * ody;
             const float one_over_d3y = one_over_d2y * ody;
 
-            const float dd_by_dx = (-d_right_right + 8.f * d_right - 8.f * d_left + d_left_left) * odx / 12.f;
-            const float dd_by_dy = (-d_up_up + 8.f * d_up - 8.f * d_down + d_down_down) * ody / 12.f;
+            const float dd_by_dx = cin.ddx;
+            const float dd_by_dy = cin.ddy;
             const float eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
             const float eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
             const float eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
@@ -377,15 +408,8 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 
             // level-0 coefficients of the x-line (row y) and y-line (column x) through this cell
             float Ax, Bx, Cx, Ay, By, Cy;
-            {
-                const float cs = p.coef_sl;
-                const float e0 = fmaxr(0.f, cs - BAll), e1 = fmaxr(0.f, cs - BAl), e2 = fmaxr(0.f, cs - BAc);
-                const float e3 = fmaxr(0.f, cs - BAr), e4 = fmaxr(0.f, cs - BArr);
-                line_coef(p.kcoef, p.dx2, e0, e1, e2, e3, e4, x == 2, x == p.nx + 1, Ax, Bx, Cx);
-                const float f0 = fmaxr(0.f, cs - BA_[0]), f1 = fmaxr(0.f, cs - BA_[1]);
-                const float f3 = fmaxr(0.f, cs - BA_[3]), f4 = fmaxr(0.f, cs - BA_[4]);
-                line_coef(p.kcoef, p.dy2, f0, f1, e2, f3, f4, y == 2, y == p.ny + 1, Ay, By, Cy);
-            }
+            line_abc(cin.Kx, cin.Qx, x == 2, x == p.nx + 1, Ax, Bx, Cx);
+            line_abc(cin.Ky, cin.Qy, y == 2, y == p.ny + 1, Ay, By, Cy);
 
             const float dt = p.dt;
             float wOut, huOut, hvOut;
@@ -394,10 +418,10 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
                 huOut = Ax * hul + Bx * huc + Cx * hur + dt * dg;
                 hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt * db;
             } else {
-                const float o0 = ld32(p.Go[0], i), o1 = ld32(p.Go[1], i), o2 = ld32(p.Go[2], i);
-                const float oo0 = ld32(p.Goo[0], i), oo1 = ld32(p.Goo[1], i), oo2 = ld32(p.Goo[2], i);
-                const float Fo = ld32(p.FGo[0], i), Go_ = ld32(p.FGo[1], i);
-                const float Foo = ld32(p.FGoo[0], i), Goo_ = ld32(p.FGoo[1], i);
+                const float o0 = cin.o0, o1 = cin.o1, o2 = cin.o2;
+                const float oo0 = cin.oo0, oo1 = cin.oo1, oo2 = cin.oo2;
+                const float Fo = cin.Fo, Go_ = cin.Go;
+                const float Foo = cin.Foo, Goo_ = cin.Goo;
                 if (SCHEME == 1) {
                     wOut = wc + dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
                     huOut = Ax * hul + Bx * huc + Cx * hur + dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1) +
@@ -444,7 +468,6 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
         }
         // ---- shift windows
         w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];
-        BA_[0] = BA_[1]; BA_[1] = BA_[2]; BA_[2] = BA_[3]; BA_[3] = BA_[4];
         hu_[0] = hu_[1]; hu_[1] = hu_[2]; hu_[2] = hu_[3];
         hv_[0] = hv_[1]; hv_[1] = hv_[2]; hv_[2] = hv_[3];
         BY_[0] = BY_[1]; BY_[1] = BY_[2]; BY_[2] = BY_[3];
@@ -553,51 +576,55 @@ __device__ __forceinline__ Eq reduce_eq(const Eq &L, const Eq &H, const Eq &R)
     return o;
 }
 
-// LDS staging index with one pad word per 32: lane chunks of C >= 32 words start in
-// different banks.
-__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
-
 template <int C, int W>
 struct LineSmem {
-    float D[64 * W * C + 2 * W * C + 8];      // padded right-hand side / solution staging
-    float B[64 * W * C + 2 * W * C + 8];      // padded depths max(0, sl - BA)
     Eq top[64 * W];                            // one unknown per thread above the chunk levels
     Eq xch[2][64 * W];                         // chunk-level neighbour exchange (double-buffered)
 };
 
+// Loads C consecutive floats starting at a 16-byte aligned element (C % 4 == 0) or singly.
+template <int C>
+__device__ __forceinline__ void load_chunk(const float *src, int base, int n, float *v)
+{
+    if (C % 4 == 0 && base + C <= n) {
+#pragma unroll
+        for (int i = 0; i < C; i += 4) {
+            const float4 f = *reinterpret_cast<const float4 *>(src + base + i);
+            v[i] = f.x; v[i + 1] = f.y; v[i + 2] = f.z; v[i + 3] = f.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < C; ++i) v[i] = base + i < n ? src[base + i] : 0.f;
+    }
+}
+
 // Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
+// D, K, Q point at element 0 of the line (16-byte aligned); out at element 0 of the
+// ghost-inclusive output line + 2 (8-byte aligned).
 template <int C, int W>
-__device__ void cr_line_block(int n, const float *D, const float *BAl, float *out, float kcoef, float del2, float cs,
-                              LineSmem<C, W> &sm)
+__device__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out, LineSmem<C, W> &sm)
 {
     constexpr int NT = 64 * W;
     constexpr int K0 = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : C == 16 ? 4 : 5;
-    constexpr int KT = W == 1 ? 6 : W == 2 ? 7 : W == 4 ? 8 : 9;     // log2(NT)
+    constexpr int KT = W == 1 ? 6 : W == 2 ? 7 : W == 4 ? 8 : W == 8 ? 9 : 10;     // log2(NT)
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16, "waves per line");
     const int t = threadIdx.x;
     const int base = t * C;
-    // coalesced loads of the right-hand side and the bottom line into LDS
-    for (int i = t; i < n; i += NT) sm.D[pad32(i)] = D[i];
-    for (int i = t; i < n + 4; i += NT) sm.B[pad32(i)] = fmaxr(0.f, cs - BAl[i]);
-    __syncthreads();
     Eq q[C];
-    // level-0 coefficients from the depths at offsets -2..+2 (engine.cpp:2656-2680)
     {
-        float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
-        if (base < n + 4) e0 = sm.B[pad32(base)];
-        if (base + 1 < n + 4) e1 = sm.B[pad32(base + 1)];
-        if (base + 2 < n + 4) e2 = sm.B[pad32(base + 2)];
-        if (base + 3 < n + 4) e3 = sm.B[pad32(base + 3)];
+        float dv[C], kv[C], qv[C];
+        load_chunk<C>(D, base, n, dv);
+        load_chunk<C>(K, base, n, kv);
+        load_chunk<C>(Qf, base, n, qv);
 #pragma unroll
         for (int i = 0; i < C; ++i) {
             const int el = base + i;
-            const float e4 = el + 4 < n + 4 ? sm.B[pad32(el + 4)] : 0.f;
             if (el < n) {
-                line_coef(kcoef, del2, e0, e1, e2, e3, e4, el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
-                q[i].d = sm.D[pad32(el)];
+                line_abc(kv[i], qv[i], el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
+                q[i].d = dv[i];
             } else {
                 q[i] = Eq{0.f, 1.f, 0.f, 0.f};
             }
-            e0 = e1; e1 = e2; e2 = e3; e3 = e4;
         }
     }
     const Eq margin{0.f, 1.f, 0.f, 0.f};
@@ -660,11 +687,14 @@ __device__ void cr_line_block(int n, const float *D, const float *BAl, float *ou
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
+    if (C % 2 == 0 && base + C <= n) {
 #pragma unroll
-    for (int i = 0; i < C; ++i)
-        if (base + i < n) sm.D[pad32(base + i)] = q[i].d;
-    __syncthreads();
-    for (int i = t; i < n; i += NT) out[i] = sm.D[pad32(i)];
+        for (int i = 0; i < C; i += 2) *reinterpret_cast<float2 *>(out + base + i) = make_float2(q[i].d, q[i + 1].d);
+    } else {
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+            if (base + i < n) out[base + i] = q[i].d;
+    }
 }
 
 template <int C, int W>
@@ -688,12 +718,13 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     __shared__ LineSmem<C, W> sm;
     const bool xdir = bid < p.nlines_x;
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
-    if (xdir)
-        cr_line_block<C, W>(p.nx, p.Dx + (size_t)line * p.P, p.BA + (size_t)line * p.P,
-                            p.hu_out + (size_t)line * p.P + 2, p.kcoef, p.dx2, p.coef_sl, sm);
-    else
-        cr_line_block<C, W>(p.ny, p.DyT + (size_t)line * p.PT, p.BAT + (size_t)line * p.PT,
-                            p.hvT_out + (size_t)line * p.PT + 2, p.kcoef, p.dy2, p.coef_sl, sm);
+    if (xdir) {
+        const size_t o = (size_t)line * p.P;
+        cr_line_block<C, W>(p.nx, p.Dx + o, p.Kx + o, p.Qx + o, p.hu_out + o + 2, sm);
+    } else {
+        const size_t o = (size_t)line * p.PT;
+        cr_line_block<C, W>(p.ny, p.DyT + o, p.KyT + o, p.QyT + o, p.hvT_out + o + 2, sm);
+    }
 }
 
 // ------------------------------------------------------------------ boundaries
@@ -842,29 +873,54 @@ __global__ void transpose_plane(const float *in, int pin, float *out, int pout, 
     }
 }
 
-// level-0 coefficients of every cell for the x- and y-lines (diagnostic export)
-__global__ void export_coefficients(const LineParams p, const float *BA, const float *BAT, float *ax, float *ay)
+// Static fields of every interior cell from the bottom (engine.cpp:2656-2680 for K1, K2/72;
+// compute.hlsl:696-697 for the depth gradients).
+__global__ void static_fields(const StaticParams p)
+{
+    const int x = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = 2 + blockIdx.y;
+    if (x >= p.nx + 2 || y >= p.ny + 2) return;
+    const float *BA = p.BA;
+    auto ba = [&](int xx, int yy) { return BA[(size_t)yy * p.P + xx]; };
+    const float cs = p.coef_sl, sl = p.sea_level;
+    float Kx, Qx, Ky, Qy;
+    line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ba(x - 2, y)), fmaxr(0.f, cs - ba(x - 1, y)), fmaxr(0.f, cs - ba(x, y)),
+            fmaxr(0.f, cs - ba(x + 1, y)), fmaxr(0.f, cs - ba(x + 2, y)), Kx, Qx);
+    line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - ba(x, y - 2)), fmaxr(0.f, cs - ba(x, y - 1)), fmaxr(0.f, cs - ba(x, y)),
+            fmaxr(0.f, cs - ba(x, y + 1)), fmaxr(0.f, cs - ba(x, y + 2)), Ky, Qy);
+    const float d_l = fmaxr(0.f, sl - ba(x - 1, y)), d_r = fmaxr(0.f, sl - ba(x + 1, y));
+    const float d_ll = fmaxr(0.f, sl - ba(x - 2, y)), d_rr = fmaxr(0.f, sl - ba(x + 2, y));
+    const float d_d = fmaxr(0.f, sl - ba(x, y - 1)), d_u = fmaxr(0.f, sl - ba(x, y + 1));
+    const float d_dd = fmaxr(0.f, sl - ba(x, y - 2)), d_uu = fmaxr(0.f, sl - ba(x, y + 2));
+    const float ddx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * p.one_over_dx / 12.f;
+    const float ddy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * p.one_over_dy / 12.f;
+    const size_t r = (size_t)y * p.P + (x - 2), c = (size_t)x * p.PT + (y - 2);
+    p.Kx[r] = Kx; p.Qx[r] = Qx; p.Ky[r] = Ky; p.Qy[r] = Qy; p.DDx[r] = ddx; p.DDy[r] = ddy;
+    p.KyT[c] = Ky; p.QyT[c] = Qy;
+}
+
+// level-0 coefficients of every cell for the x- and y-lines (diagnostic export): ghost rows
+// of the x-lines repeat the nearest interior row, ghost columns of the y-lines the nearest
+// interior column (engine.cpp:2641, 2797); line margins are (0, 1, 0).
+__global__ void export_coefficients(const StaticParams p, float *ax, float *ay)
 {
     const int X = p.nx + 4, Y = p.ny + 4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= X * Y) return;
     const int x = i % X, y = i / X;
     const size_t N = (size_t)X * Y;
-    const float cs = p.coef_sl;
     float A = 0.f, B = 1.f, C = 0.f;
     if (x >= 2 && x <= X - 3) {
         const int yy = y < 2 ? 2 : (y > p.ny + 1 ? p.ny + 1 : y);
-        const float *r = BA + (size_t)yy * p.P;
-        line_coef(p.kcoef, p.dx2, fmaxr(0.f, cs - r[x - 2]), fmaxr(0.f, cs - r[x - 1]), fmaxr(0.f, cs - r[x]),
-                  fmaxr(0.f, cs - r[x + 1]), fmaxr(0.f, cs - r[x + 2]), x == 2, x == X - 3, A, B, C);
+        const size_t r = (size_t)yy * p.P + (x - 2);
+        line_abc(p.Kx[r], p.Qx[r], x == 2, x == X - 3, A, B, C);
     }
     ax[i] = A; ax[N + i] = B; ax[2 * N + i] = C;
     A = 0.f; B = 1.f; C = 0.f;
     if (y >= 2 && y <= Y - 3) {
         const int xx = x < 2 ? 2 : (x > p.nx + 1 ? p.nx + 1 : x);
-        const float *cl = BAT + (size_t)xx * p.PT;
-        line_coef(p.kcoef, p.dy2, fmaxr(0.f, cs - cl[y - 2]), fmaxr(0.f, cs - cl[y - 1]), fmaxr(0.f, cs - cl[y]),
-                  fmaxr(0.f, cs - cl[y + 1]), fmaxr(0.f, cs - cl[y + 2]), y == 2, y == Y - 3, A, B, C);
+        const size_t r = (size_t)y * p.P + (xx - 2);
+        line_abc(p.Ky[r], p.Qy[r], y == 2, y == Y - 3, A, B, C);
     }
     ay[i] = A; ay[N + i] = B; ay[2 * N + i] = C;
 }
@@ -909,7 +965,11 @@ hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_
         else if (nmax <= 256) CEL_SOLVE(4, 1);
         else if (nmax <= 512) CEL_SOLVE(8, 1);
         else if (nmax <= 1024) CEL_SOLVE(8, 2);
+#ifdef CEL_SOLVE_2048
+        else if (nmax <= 2048) CEL_SOLVE_2048;
+#else
         else if (nmax <= 2048) CEL_SOLVE(8, 4);
+#endif
         else if (nmax <= 4096) CEL_SOLVE(16, 4);
         else CEL_SOLVE(16, 8);
 #undef CEL_SOLVE
@@ -926,11 +986,16 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_export_coefficients(const LineParams &p, const float *BA, const float *BAT, float *ax, float *ay,
-                                      hipStream_t st)
+hipError_t launch_static_fields(const StaticParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(static_fields, dim3((p.nx + 127) / 128, p.ny), dim3(128), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st)
 {
     const int n = (p.nx + 4) * (p.ny + 4);
-    hipLaunchKernelGGL(export_coefficients, dim3((n + 255) / 256), dim3(256), 0, st, p, BA, BAT, ax, ay);
+    hipLaunchKernelGGL(export_coefficients, dim3((n + 255) / 256), dim3(256), 0, st, p, ax, ay);
     return hipGetLastError();
 }
 

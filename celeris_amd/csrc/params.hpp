@@ -39,17 +39,28 @@ struct ExplicitParams {
     // variable-step AB3 coefficients (AdamsBashforthCoef) and derivative weights
     float ab[3];
     float der[9];
-    // level-0 tridiagonal coefficients: (BCOEF + 1/3), dx^2, dy^2 and their sea level
-    float kcoef, dx2, dy2, coef_sl;
+    // static per-cell fields, [y * P + (x - 2)]: K1 and K2/72 of the x- and y-line
+    // coefficients, and the fourth-order depth gradients dd/dx, dd/dy
+    const float *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;
+};
+
+// Static per-cell fields derived from the bottom once per mesh (bit-identical to computing
+// them inside the step: same operations on the same inputs).
+struct StaticParams {
+    int nx, ny, P, PT;
+    const float *BA;
+    float *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;   // row-major, [y * P + (x - 2)]
+    float *KyT, *QyT;                       // transposed, [x * PT + (y - 2)]
+    float kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
+    float sea_level, one_over_dx, one_over_dy;
 };
 
 struct LineParams {
     int nx, ny, P, PT;
     int nlines_x, nlines_y;         // blocks [0, nlines_x) solve x-lines, then y-lines
-    const float *Dx, *BA;           // x-lines: rhs row-major, bottom centre row-major
-    const float *DyT, *BAT;         // y-lines: rhs and bottom centre transposed
+    const float *Dx, *Kx, *Qx;      // x-lines: rhs and coefficient fields, [y * P + (x - 2)]
+    const float *DyT, *KyT, *QyT;   // y-lines: the same, transposed, [x * PT + (y - 2)]
     float *hu_out, *hvT_out;
-    float kcoef, dx2, dy2, coef_sl;
     int Lx, Ly;                     // cyclic-reduction level counts
     // irregular-wave strips, handled by blocks after the line solves
     int n_irr_lines;                // total (side, strip-index, line) triples

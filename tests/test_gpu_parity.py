@@ -154,3 +154,16 @@ def test_lines_longer_than_a_wave_chunk():
     sc = S.small(nx=2600, ny=12, seed=14)
     a, b = run_pair(sc, 30)
     check(sc, a, b, exact=True)
+
+
+def test_level0_coefficients_match_oracle():
+    """A, B, C of both line directions, ghost rows/columns and margins included."""
+    for kw in (dict(nx=37, ny=29, seed=3), dict(nx=40, ny=30, seed=4, is_boussinesq=0), dict(nx=33, ny=41, seed=5,
+                                                                                          still_water=0.2)):
+        sc = S.small(**kw)
+        eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        ax, ay = eng.coefficients()
+        eng.close()
+        o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
+        bx, by = o.coefficients()
+        assert np.array_equal(ax, bx) and np.array_equal(ay, by)
