@@ -965,11 +965,7 @@ hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_
         else if (nmax <= 256) CEL_SOLVE(4, 1);
         else if (nmax <= 512) CEL_SOLVE(8, 1);
         else if (nmax <= 1024) CEL_SOLVE(8, 2);
-#ifdef CEL_SOLVE_2048
-        else if (nmax <= 2048) CEL_SOLVE_2048;
-#else
         else if (nmax <= 2048) CEL_SOLVE(8, 4);
-#endif
         else if (nmax <= 4096) CEL_SOLVE(16, 4);
         else CEL_SOLVE(16, 8);
 #undef CEL_SOLVE
