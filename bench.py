@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--graph-steps", type=int, default=50,
+                    help="steps per captured hipGraph block (0: launch step by step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (reference for HBM byte counters)")
@@ -78,6 +80,7 @@ def main():
     sc = S.headline(nx=args.nx, ny=args.ny)
     eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     cells = sc.s.nx * sc.s.ny
+    eng.set_graph_steps(args.graph_steps)
     eng.timestep(args.warmup)
     eng.sync()
 
@@ -116,7 +119,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Boussinesq step, {sc.s.nx}x{sc.s.ny} interior cells, dx 1 m, 1:30 slope + GRF, "
                                "west irregular waves (100 JONSWAP components), 20-cell east sponge, N/S walls, FP32",
-                   "nx": sc.s.nx, "ny": sc.s.ny, "dt": sc.s.timestep,
+                   "nx": sc.s.nx, "ny": sc.s.ny, "dt": sc.s.timestep, "graph_steps": args.graph_steps,
                    "hbm_frac_step": value * 1e9 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
                    "event_ms_per_step": ms_events / args.steps,
                    "kernel_avg_ms": kern, "state_finite": finite},

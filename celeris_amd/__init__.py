@@ -40,6 +40,20 @@ class CelSoliton(ctypes.Structure):
                 ("xc", ctypes.c_float), ("yc", ctypes.c_float)]
 
 
+class CelStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "sum_h", "sum_bh_h2", "sum_hu", "sum_hv", "sum_hu2v2", "max_u2v2", "max_h", "max_cfl_rate", "max_u2v2_h",
+        "mass", "x_momentum", "y_momentum", "kinetic_energy", "potential_energy", "total_energy",
+        "max_speed", "max_depth", "max_froude", "cfl_number")] + [("timestep", ctypes.c_float), ("cells", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class CelAdaptive(ctypes.Structure):
+    _fields_ = [("ema_alpha", ctypes.c_float), ("min_timestep_ratio", ctypes.c_float), ("target_cfl", ctypes.c_float)]
+
+
 # Every symbol include/celeris.h declares, with its argument types.
 SIGNATURES = {
     "cel_last_error": ([], ctypes.c_char_p),
@@ -61,6 +75,10 @@ SIGNATURES = {
     "cel_engine_kernel_time": ([ctypes.c_void_p, ctypes.c_char_p, _F, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_engine_device_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "cel_engine_coefficients": ([ctypes.c_void_p, _F, _F], ctypes.c_int),
+    "cel_engine_set_graph_steps": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_engine_stats": ([ctypes.c_void_p, ctypes.POINTER(CelStats)], ctypes.c_int),
+    "cel_engine_reset_timestep": ([ctypes.c_void_p, ctypes.POINTER(CelAdaptive), ctypes.POINTER(CelStats)],
+                                  ctypes.c_int),
 }
 
 _lib = None
@@ -224,6 +242,23 @@ class ShallowWaterEngine:
         b = ctypes.c_size_t()
         _check(load_library().cel_engine_device_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    def set_graph_steps(self, k):
+        """Run cel_engine_step in captured blocks of k steps (0: step by step)."""
+        _check(load_library().cel_engine_set_graph_steps(self._h, k))
+
+    def stats(self):
+        """GetStats of the current state (FP64 device reductions) as a dict."""
+        st = CelStats()
+        _check(load_library().cel_engine_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def update_timestep(self, ema_alpha=0.5, min_timestep_ratio=0.05, target_cfl=0.0):
+        """resetTimestep: GetStats, then the dt of the next step (adaptive or uniform)."""
+        st = CelStats()
+        a = CelAdaptive(ema_alpha, min_timestep_ratio, target_cfl)
+        _check(load_library().cel_engine_reset_timestep(self._h, ctypes.byref(a), ctypes.byref(st)))
+        return st.as_dict()
 
     def coefficients(self):
         ax = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)

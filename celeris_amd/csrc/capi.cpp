@@ -160,4 +160,18 @@ int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3)
     });
 }
 
+int cel_engine_set_graph_steps(cel_engine *e, int steps) { return guarded([&] { eng(e).setGraphSteps(steps); }); }
+
+int cel_engine_stats(cel_engine *e, cel_stats *out) { return guarded([&] { eng(e).stats(out); }); }
+
+int cel_engine_reset_timestep(cel_engine *e, const cel_adaptive *a, cel_stats *out)
+{
+    return guarded([&] {
+        if (!a) throw cel::Error(CEL_EINVAL, "null adaptive settings");
+        if (!(a->ema_alpha >= 0.f && a->ema_alpha <= 1.f)) throw cel::Error(CEL_EINVAL, "ema_alpha must lie in [0, 1]");
+        if (!(a->min_timestep_ratio > 0.f)) throw cel::Error(CEL_EINVAL, "min_timestep_ratio must be positive");
+        eng(e).resetTimestepAdaptive(*a, out);
+    });
+}
+
 }  // extern "C"

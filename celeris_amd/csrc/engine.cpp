@@ -71,6 +71,7 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
         side_[i].cfg = bnd[i];
     }
     current_timestep_ = old_timestep_ = old_old_timestep_ = s.timestep;
+    uniform_dt_ = s.timestep;
     check(hipGetDevice(&device_), "hipGetDevice");
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
@@ -85,6 +86,12 @@ ShallowWaterEngine::~ShallowWaterEngine()
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (int i = 0; i < 4; ++i) freeSideTables(i);
     if (arena_) (void)hipFree(arena_);
+    if (d_stats_) (void)hipFree(d_stats_);
+    if (h_stats_) (void)hipHostFree(h_stats_);
+    clearGraphs();
+    if (d_clock_) (void)hipFree(d_clock_);
+    for (auto &h : h_clock_) if (h) (void)hipHostFree(h);
+    for (auto &e : clock_ev_) if (e) (void)hipEventDestroy(e);
     for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
     for (auto &e : pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -113,6 +120,14 @@ void ShallowWaterEngine::allocate()
     Cw_ = rp(); Chu_ = rp();
     Kx_ = rp(); Qx_ = rp(); Ky_ = rp(); Qy_ = rp(); DDx_ = rp(); DDy_ = rp();
     hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); KyT_ = cp(); QyT_ = cp();
+    const int tiles = ((4 * (nx_ / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (ny_ / 4) + kStatsTileY - 1) / kStatsTileY);
+    stats_blocks_ = tiles;
+    const size_t sbytes = ((size_t)tiles + 1) * kStatsValues * sizeof(double);
+    check(hipMalloc(&mem, sbytes), "hipMalloc(stats)");
+    d_stats_ = static_cast<double *>(mem);
+    table_bytes_ += sbytes;
+    check(hipHostMalloc(&mem, kStatsValues * sizeof(double), hipHostMallocDefault), "hipHostMalloc(stats)");
+    h_stats_ = static_cast<double *>(mem);
 }
 
 void ShallowWaterEngine::uploadBottom(const float *bottom3)
@@ -223,6 +238,7 @@ void ShallowWaterEngine::setBoundary(int side, const cel_boundary &b)
     sd.cfg.waves = sd.waves.empty() ? nullptr : sd.waves.data();
     sd.cfg.uts = sd.uts.empty() ? nullptr : sd.uts.data();
     sd.uts_iter = 1;
+    clearGraphs();
     buildSideTables(side);
 }
 
@@ -410,9 +426,11 @@ void ShallowWaterEngine::buildSideTables(int side)
 void ShallowWaterEngine::resetTimestep(float dt)
 {
     if (!(dt > 0.f)) throw Error(CEL_EINVAL, "dt must be positive");
+    uniform_dt_ = dt;
     if (dt != current_timestep_) {
         check(hipStreamSynchronize(stream_), "sync");
         current_timestep_ = dt;
+        clearGraphs();
         for (int i = 0; i < 4; ++i)
             if (side_[i].cfg.type == CEL_SPONGE) buildSideTables(i);
     }
@@ -509,42 +527,38 @@ void ShallowWaterEngine::fillLines(LineParams &p) const
     p.Lx = Lx_; p.Ly = Ly_;
 }
 
-void ShallowWaterEngine::timestep()
+// Everything one step issues, from the host state at the start of the step
+// (fillConstantBuffers + the pass setup of engine.cpp:849-1438); then the host state
+// advances as after the step (engine.cpp:1476-1521).
+void ShallowWaterEngine::planStep(StepLaunch &L)
 {
+    std::memset(&L, 0, sizeof(L));
     const int scheme = time_scheme_now_;
     const int n = 1 - cur_;
-    ExplicitParams ep;
-    fillExplicit(ep);
-    hipEvent_t pe[4] = {};
-    if (profiling_) {
-        for (auto &e : pe) e = nextEvent();
-        check(hipEventRecord(pe[0], stream_), "event");
-    }
-    check(launch_explicit(scheme, ep, stream_), "explicit_step");
-    if (profiling_) check(hipEventRecord(pe[1], stream_), "event");
+    L.scheme = scheme;
+    fillExplicit(L.ep);
     // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
     // steps nothing ever reads them
     if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
         const size_t plane = (size_t)P_ * Y_ * 4, tplane = (size_t)PT_ * X_ * 4;
-        for (int k = 0; k < 3; ++k) check(hipMemcpyAsync(Gp_[k], G_[ring_scr_][k], plane, hipMemcpyDeviceToDevice, stream_), "copy");
-        for (int k = 0; k < 2; ++k) check(hipMemcpyAsync(FGp_[k], FG_[ring_scr_][k], plane, hipMemcpyDeviceToDevice, stream_), "copy");
-        check(hipMemcpyAsync(Cw_, w_[cur_], plane, hipMemcpyDeviceToDevice, stream_), "copy");
-        check(hipMemcpyAsync(Chu_, hu_[cur_], plane, hipMemcpyDeviceToDevice, stream_), "copy");
-        check(hipMemcpyAsync(ChvT_, hvT_[cur_], tplane, hipMemcpyDeviceToDevice, stream_), "copy");
+        auto add = [&](float *d, const float *src, size_t bytes) { L.copy[L.ncopy++] = StepLaunch::Copy{d, src, bytes}; };
+        for (int k = 0; k < 3; ++k) add(Gp_[k], G_[ring_scr_][k], plane);
+        for (int k = 0; k < 2; ++k) add(FGp_[k], FG_[ring_scr_][k], plane);
+        add(Cw_, w_[cur_], plane);
+        add(Chu_, hu_[cur_], plane);
+        add(ChvT_, hvT_[cur_], tplane);
     }
     // line solves + irregular-wave lines
-    LineParams lp;
-    fillLines(lp);
-    IrregularAll irr;
-    std::memset(&irr, 0, sizeof(irr));
+    fillLines(L.lp);
+    IrregularAll &irr = L.irr;
     irr.t = total_time_;
     for (int s = 0; s < 4; ++s) {
         const SideState &sd = side_[s];
         if (sd.cfg.type != CEL_IRREGULARWAVES || sd.cfg.width == 0) continue;
         const bool xs = s == CEL_WEST || s == CEL_EAST;
         IrregularSide &S = irr.side[s];
-        const int L = xs ? Lx_ : Ly_;
-        const int pn = 1 << (L - 1);
+        const int Lv = xs ? Lx_ : Ly_;
+        const int pn = 1 << (Lv - 1);
         S.active = 1;
         S.width = sd.cfg.width;
         S.nlines = xs ? Y_ : X_;
@@ -555,20 +569,15 @@ void ShallowWaterEngine::timestep()
         int above = 0;
         for (int v = p2; v < pn; v <<= 1) ++above;
         S.levels_above = above;
-        S.zero_line = L == 1;
+        S.zero_line = Lv == 1;
         S.sl = sd.cfg.water_level;
         const int nw1 = std::max(sd.irr_nw, 1);
         S.amp = sd.d_irr_wave; S.omega = sd.d_irr_wave + nw1; S.phase = sd.d_irr_wave + 2 * nw1;
-        S.tab = sd.d_tab; S.wet = sd.d_wet; S.nw = nw1 == 0 ? 1 : sd.irr_nw; S.out = sd.d_irr_out;
-        if (sd.irr_nw == 0) S.nw = 1;
+        S.tab = sd.d_tab; S.wet = sd.d_wet; S.nw = nw1; S.out = sd.d_irr_out;
         irr.count[s] = S.width * S.nlines;
     }
-    if (profiling_) check(hipEventRecord(pe[2], stream_), "event");
-    check(launch_solve(lp, irr, stream_), "solve_lines");
-    if (profiling_) check(hipEventRecord(pe[3], stream_), "event");
     // boundaries (engine.cpp:1235-1438), boundary time = t^n (engine.cpp:1324)
-    BoundaryParams bp;
-    std::memset(&bp, 0, sizeof(bp));
+    BoundaryParams &bp = L.bp;
     bp.nx = nx_; bp.ny = ny_; bp.P = P_; bp.PT = PT_; bp.X = X_; bp.Y = Y_;
     bp.band_l = std::max(side_[CEL_WEST].cfg.width, 2);
     bp.band_r = std::max(side_[CEL_EAST].cfg.width, 2);
@@ -577,21 +586,17 @@ void ShallowWaterEngine::timestep()
     bp.ncells = (bp.band_l + bp.band_r) * Y_ + (X_ - bp.band_l - bp.band_r) * (bp.band_b + bp.band_t);
     bp.t = total_time_;
     bp.w = w_[n]; bp.hu = hu_[n]; bp.hvT = hvT_[n];
+    L.clock[0] = total_time_;
     for (int s = 0; s < 4; ++s) {
         SideParams &S = bp.side[s];
         const SideState &sd = side_[s];
         S.type = sd.cfg.type; S.width = sd.cfg.width; S.sl = sd.cfg.water_level;
         S.tab = sd.d_tab; S.amp = sd.cfg.amplitude; S.omega = sd.sine_omega;
         S.irr = sd.d_irr_out;
-        if (S.type == CEL_UNIFORMTIMESERIES) timeSeries(s, total_time_, S.uts_eta, S.uts_hu, S.uts_hv);
-    }
-    bp.mirror_pass = 1;
-    check(launch_boundary(bp, stream_), "boundary_apply");
-    bp.mirror_pass = 0;
-    check(launch_boundary(bp, stream_), "boundary_apply");
-    if (profiling_) {
-        hipEvent_t e4 = nextEvent();
-        check(hipEventRecord(e4, stream_), "event");
+        if (S.type == CEL_UNIFORMTIMESERIES) {
+            timeSeries(s, total_time_, S.uts_eta, S.uts_hu, S.uts_hv);
+            L.clock[1 + 3 * s] = S.uts_eta; L.clock[2 + 3 * s] = S.uts_hu; L.clock[3 + 3 * s] = S.uts_hv;
+        }
     }
     // time-scheme state machine for the next step (engine.cpp:1476-1521)
     if (timestep_count_ >= 2) {
@@ -616,10 +621,132 @@ void ShallowWaterEngine::timestep()
     cur_ = n;
 }
 
+// Issues one planned step on the stream.  With dclock set, the kernels take t^n and the
+// time-series values from that device entry instead of their parameter blocks.
+void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, bool profile)
+{
+    hipEvent_t pe[4] = {};
+    if (profile) {
+        for (auto &e : pe) e = nextEvent();
+        check(hipEventRecord(pe[0], stream_), "event");
+    }
+    check(launch_explicit(L.scheme, L.ep, stream_), "explicit_step");
+    if (profile) check(hipEventRecord(pe[1], stream_), "event");
+    for (int i = 0; i < L.ncopy; ++i)
+        check(hipMemcpyAsync(L.copy[i].dst, L.copy[i].src, L.copy[i].bytes, hipMemcpyDeviceToDevice, stream_), "copy");
+    IrregularAll irr = L.irr;
+    BoundaryParams bp = L.bp;
+    irr.clock = dclock;
+    bp.clock = dclock;
+    if (profile) check(hipEventRecord(pe[2], stream_), "event");
+    check(launch_solve(L.lp, irr, stream_), "solve_lines");
+    if (profile) check(hipEventRecord(pe[3], stream_), "event");
+    bp.mirror_pass = 1;
+    check(launch_boundary(bp, stream_), "boundary_apply");
+    bp.mirror_pass = 0;
+    check(launch_boundary(bp, stream_), "boundary_apply");
+    if (profile) {
+        hipEvent_t e4 = nextEvent();
+        check(hipEventRecord(e4, stream_), "event");
+    }
+}
+
+void ShallowWaterEngine::timestep()
+{
+    StepLaunch L;
+    planStep(L);
+    issueStep(L, nullptr, profiling_);
+}
+
+// ---- captured multi-step blocks
+//
+// Every kernel parameter of a step is a function of the discrete engine state (ping-pong
+// index, history ring, scheme counters, dt history) except the step clock.  A block of K
+// steps is captured once per starting discrete state; each replay first copies that
+// block's clock table (computed by planning the K steps on the host) to the device.
+
+void ShallowWaterEngine::setGraphSteps(int k)
+{
+    if (k < 0 || k > kMaxGraphSteps) throw Error(CEL_EINVAL, "steps per graph must lie in [0, 1024]");
+    sync();
+    clearGraphs();
+    if (k > 0 && !d_clock_) {
+        void *mem = nullptr;
+        check(hipMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(float)), "hipMalloc(clock)");
+        d_clock_ = static_cast<float *>(mem);
+        for (auto &h : h_clock_) {
+            check(hipHostMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(float), hipHostMallocDefault),
+                  "hipHostMalloc(clock)");
+            h = static_cast<float *>(mem);
+        }
+        for (auto &e : clock_ev_) check(hipEventCreate(&e), "hipEventCreate");
+        table_bytes_ += (size_t)kMaxGraphSteps * kClockStride * sizeof(float);
+    }
+    graph_steps_ = k;
+}
+
+void ShallowWaterEngine::clearGraphs()
+{
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
+ShallowWaterEngine::GraphKey ShallowWaterEngine::discreteKey() const
+{
+    GraphKey k;
+    k.v[0] = cur_; k.v[1] = ring_old_; k.v[2] = ring_oo_; k.v[3] = ring_scr_;
+    k.v[4] = time_scheme_now_; k.v[5] = correction_steps_count_; k.v[6] = std::min(timestep_count_, 3);
+    k.v[7] = graph_steps_;
+    std::memcpy(&k.v[8], &current_timestep_, 4);
+    std::memcpy(&k.v[9], &old_timestep_, 4);
+    std::memcpy(&k.v[10], &old_old_timestep_, 4);
+    return k;
+}
+
+void ShallowWaterEngine::graphBlock()
+{
+    const int K = graph_steps_;
+    const GraphKey key = discreteKey();
+    std::vector<StepLaunch> plans(K);
+    for (int k = 0; k < K; ++k) planStep(plans[k]);
+    hipGraphExec_t exec = nullptr;
+    for (const auto &g : graphs_)
+        if (g.key == key) exec = g.exec;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            for (int k = 0; k < K; ++k) issueStep(plans[k], d_clock_ + (size_t)k * kClockStride, false);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "hipGraphInstantiate");
+        graphs_.push_back(GraphEntry{key, exec});
+    }
+    // the host clock buffer of this slot was last read by the copy two blocks ago
+    float *h = h_clock_[clock_slot_];
+    if (clock_used_[clock_slot_]) check(hipEventSynchronize(clock_ev_[clock_slot_]), "clock event");
+    for (int k = 0; k < K; ++k) std::memcpy(h + (size_t)k * kClockStride, plans[k].clock, sizeof(plans[k].clock));
+    check(hipMemcpyAsync(d_clock_, h, (size_t)K * kClockStride * sizeof(float), hipMemcpyHostToDevice, stream_),
+          "upload clock");
+    check(hipEventRecord(clock_ev_[clock_slot_], stream_), "clock event");
+    clock_used_[clock_slot_] = true;
+    clock_slot_ ^= 1;
+    check(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
+}
+
 void ShallowWaterEngine::step(int nsteps)
 {
     if (nsteps < 0) throw Error(CEL_EINVAL, "negative step count");
-    for (int i = 0; i < nsteps; ++i) timestep();
+    int done = 0;
+    if (graph_steps_ > 0 && !profiling_)
+        for (; done + graph_steps_ <= nsteps; done += graph_steps_) graphBlock();
+    for (; done < nsteps; ++done) timestep();
 }
 
 void ShallowWaterEngine::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
@@ -679,6 +806,90 @@ bool ShallowWaterEngine::kernelTime(const std::string &name, float *ms, int *lau
     if (ms) *ms = kt_[i].ms;
     if (launches) *launches = kt_[i].n;
     return true;
+}
+
+void ShallowWaterEngine::runStats(double *out9)
+{
+    StatsParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.ncx = 4 * (nx_ / 4); p.ncy = 4 * (ny_ / 4);
+    p.P = P_; p.PT = PT_;
+    p.tiles_x = (p.ncx + kStatsTileX - 1) / kStatsTileX;
+    p.nblocks = stats_blocks_;
+    p.w = w_[cur_]; p.hu = hu_[cur_]; p.hvT = hvT_[cur_]; p.BA = BA_;
+    p.g = s_.gravity;
+    p.one_over_dx = 1.0f / dx_;
+    p.one_over_dy = 1.0f / dy_;
+    const float m = std::max(dx_, dy_);
+    p.epsilon = std::min(s_.epsilon, (m * m) * (m * m));
+    p.partial = d_stats_;
+    p.out = d_stats_ + (size_t)stats_blocks_ * kStatsValues;
+    check(launch_stats(p, stream_), "stats");
+    check(hipMemcpyAsync(h_stats_, p.out, kStatsValues * sizeof(double), hipMemcpyDeviceToHost, stream_), "download stats");
+    check(hipStreamSynchronize(stream_), "stats sync");
+    std::memcpy(out9, h_stats_, kStatsValues * sizeof(double));
+}
+
+// resetTimestep's post-processing of the reductions, engine.cpp:1686-1696 (in FP64)
+void ShallowWaterEngine::deriveStats(const double *r, cel_stats *o) const
+{
+    std::memset(o, 0, sizeof(*o));
+    o->sum_h = r[0]; o->sum_bh_h2 = r[1]; o->sum_hu = r[2]; o->sum_hv = r[3]; o->sum_hu2v2 = r[4];
+    o->max_u2v2 = r[5]; o->max_h = r[6]; o->max_cfl_rate = r[7]; o->max_u2v2_h = r[8];
+    const double density = 1000.0;
+    const double area = (double)s_.width / (double)(nx_ - 1) * ((double)s_.length / (double)(ny_ - 1));
+    const double g = s_.gravity;
+    o->mass = r[0] * density * area;
+    o->x_momentum = r[2] * density * area;
+    o->y_momentum = r[3] * density * area;
+    o->kinetic_energy = 0.5 * density * area * r[4];
+    o->potential_energy = g * density * area * r[1];
+    o->total_energy = o->kinetic_energy + o->potential_energy;
+    o->max_speed = std::sqrt(r[5]);
+    o->max_depth = r[6];
+    o->max_froude = std::sqrt(r[8] / g);
+    o->timestep = current_timestep_;
+    o->cfl_number = r[7] * (double)current_timestep_;
+    o->cells = 16 * (nx_ / 4) * (ny_ / 4);
+}
+
+void ShallowWaterEngine::stats(cel_stats *out)
+{
+    if (!out) throw Error(CEL_EINVAL, "null stats");
+    double raw[kStatsValues];
+    runStats(raw);
+    deriveStats(raw, out);
+}
+
+// resetTimestep, engine.cpp:1698-1729 (dt arithmetic in float, as the reference)
+void ShallowWaterEngine::resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out)
+{
+    double raw[kStatsValues];
+    runStats(raw);
+    float cfl = (float)raw[7];
+    if (cfl > 0 && !target_cfl_set_ && s_.time_scheme == CEL_PREDICTOR_ADAPTIVE) {
+        target_cfl_ = s_.timestep * cfl;
+        target_cfl_set_ = true;
+    }
+    const float safety = target_cfl_set_ ? target_cfl_ : a.target_cfl;
+    float cur = current_timestep_;
+    if (!dt_initialized_) {
+        cur = s_.timestep;
+        cfl = safety / cur;
+        old_old_timestep_ = old_timestep_ = cur;
+        dt_initialized_ = true;
+    }
+    if (time_scheme_now_ == CEL_PREDICTOR_ADAPTIVE) {
+        const float target = std::max(safety / cfl, s_.timestep * a.min_timestep_ratio);
+        cur = target > old_timestep_ ? a.ema_alpha * target + (1 - a.ema_alpha) * old_timestep_ : target;
+    } else {
+        cur = uniform_dt_;
+    }
+    if (!(cur > 0.f) || !std::isfinite(cur)) throw Error(CEL_ENUMERIC, "adaptive dt is not a positive finite number");
+    const float keep_uniform = uniform_dt_;
+    resetTimestep(cur);
+    uniform_dt_ = keep_uniform;
+    if (out) deriveStats(raw, out);
 }
 
 void ShallowWaterEngine::coefficients(float *ax, float *ay)

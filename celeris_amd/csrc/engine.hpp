@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -28,6 +29,7 @@ hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
 hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st);
+hipError_t launch_stats(const StatsParams &p, hipStream_t st);
 
 // st_getCountOfMatrices, engine.cpp:741-748
 int count_of_matrices(int n);
@@ -45,6 +47,21 @@ struct SideState {
     float sine_omega = 0.f;
     int irr_nw = 0;
 };
+
+// Everything one step issues (kernel parameter blocks, predictor copies, step clock).
+struct StepLaunch {
+    struct Copy { float *dst; const float *src; size_t bytes; };
+    int scheme;
+    int ncopy;
+    Copy copy[8];
+    ExplicitParams ep;
+    LineParams lp;
+    IrregularAll irr;
+    BoundaryParams bp;
+    float clock[kClockStride];
+};
+
+constexpr int kMaxGraphSteps = 1024;
 
 class ShallowWaterEngine {
 public:
@@ -67,6 +84,12 @@ public:
     bool kernelTime(const std::string &name, float *ms, int *launches);
     size_t deviceBytes() const { return arena_bytes_ + table_bytes_; }
     void coefficients(float *ax, float *ay);
+    void stats(cel_stats *out);
+    void resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out);
+    void setGraphSteps(int k);                                // steps per captured block (0: off)
+    // plans the next step (kernel parameters from the current host state) and advances
+    // the host state as the step will
+    void planStep(StepLaunch &L);
 
 private:
     void allocate();
@@ -79,6 +102,24 @@ private:
     void fillLines(LineParams &p) const;
     void fillStatic(StaticParams &p) const;
     void check(hipError_t e, const char *what) const;
+    void issueStep(const StepLaunch &L, const float *dclock, bool profile);
+
+    // captured multi-step blocks
+    struct GraphKey {
+        int v[11] = {};
+        bool operator==(const GraphKey &o) const { return std::equal(v, v + 11, o.v); }
+    };
+    struct GraphEntry { GraphKey key; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs_;
+    int graph_steps_ = 0;
+    float *d_clock_ = nullptr;
+    float *h_clock_[2] = {};
+    hipEvent_t clock_ev_[2] = {};
+    bool clock_used_[2] = {};
+    int clock_slot_ = 0;
+    GraphKey discreteKey() const;
+    void graphBlock();
+    void clearGraphs();
     void timeSeries(int side, float t, float &eta, float &hu, float &hv);
 
     cel_settings s_;
@@ -100,6 +141,16 @@ private:
     float *Gp_[3], *FGp_[2];
     float *Cw_, *Chu_, *ChvT_;
     std::vector<float> hostBA_;
+
+    // GetStats buffers and resetTimestep's adaptive-dt state (engine.cpp:1698-1726)
+    double *d_stats_ = nullptr;     // partials, then the nine results
+    double *h_stats_ = nullptr;     // pinned
+    int stats_blocks_ = 0;
+    float uniform_dt_ = 0.f;
+    bool dt_initialized_ = false, target_cfl_set_ = false;
+    float target_cfl_ = 0.f;
+    void runStats(double *out9);
+    void deriveStats(const double *raw, cel_stats *out) const;
 
     // scheme state (engine.cpp:3254-3269, 1476-1519)
     int cur_ = 0;

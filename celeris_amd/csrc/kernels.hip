@@ -199,22 +199,25 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
     return c;
 }
 
+// LDS of one explicit tile: the band's transposed hv (and the corrector's start hv),
+// and the y-line right-hand sides, stored transposed after the band.
+template <bool CORRECTOR>
+struct ExplicitSmem {
+    float hv[kBandRows + 4][64];
+    float chv[CORRECTOR ? kBandRows + 4 : 1][64];
+    float dy[kBandRows][64];
+};
+
+// One wave computes interior columns [x0, x0 + 60) of rows [ys, ys + R) of one scenario.
 // SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
-template <int SCHEME>
-__global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
+template <int SCHEME, bool CS>
+__device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS> &sm)
 {
     constexpr int R = kBandRows;
-    __shared__ float hvblk[R + 4][64];
-    __shared__ float chvblk[SCHEME == 2 ? (R + 4) : 1][64];
-    __shared__ float dystg[R][64];      // y-line right-hand sides, stored transposed after the band
-
+    float (*hvblk)[64] = sm.hv;
+    float (*chvblk)[64] = sm.chv;
+    float (*dystg)[64] = sm.dy;
     const int lane = threadIdx.x;
-    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of
-    // tiles (strip-fastest) so neighbouring strips and bands share that XCD's L2
-    const int ntiles = p.nstrips * p.nbands;
-    const int per_xcd = (ntiles + 7) / 8;
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;
     const int strip = tile % p.nstrips;
     const int band = tile / p.nstrips;
     const int X = p.nx + 4;
@@ -488,6 +491,19 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     }
 }
 
+template <int SCHEME>
+__global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
+{
+    __shared__ ExplicitSmem<SCHEME == 2> sm;
+    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of
+    // tiles (strip-fastest) so neighbouring strips and bands share that XCD's L2
+    const int ntiles = p.nstrips * p.nbands;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    explicit_tile<SCHEME>(p, tile, sm);
+}
+
 // ------------------------------------------------------------------ line solves
 
 
@@ -697,25 +713,29 @@ __device__ void cr_line_block(int n, const float *D, const float *K, const float
     }
 }
 
+// One block: an x-line, a y-line, or W irregular-wave lines.  (The body stays in the
+// kernel: routing the by-value parameter blocks through a device function makes the
+// compiler copy them to scratch.)
 template <int C, int W>
 __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, const IrregularAll irr)
 {
+    __shared__ LineSmem<C, W> sm;
     const int bid = blockIdx.x;
     const int nlines = p.nlines_x + p.nlines_y;
     if (bid >= nlines) {
-        // irregular-wave lines: W per block (one per wave)
+        // irregular-wave lines: W per block (one per wave); time t^n from the step clock
+        const float t = irr.clock ? irr.clock[0] : irr.t;
         int idx = (bid - nlines) * W + (threadIdx.x >> 6);
         for (int s = 0; s < 4; ++s) {
             if (idx < irr.count[s]) {
                 const IrregularSide &S = irr.side[s];
-                irregular_line(S, idx / S.nlines, idx % S.nlines, irr.t);
+                irregular_line(S, idx / S.nlines, idx % S.nlines, t);
                 return;
             }
             idx -= irr.count[s];
         }
         return;
     }
-    __shared__ LineSmem<C, W> sm;
     const bool xdir = bid < p.nlines_x;
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     if (xdir) {
@@ -764,10 +784,9 @@ __device__ __forceinline__ Val pre_value(const BoundaryParams &p, int x, int y)
     return v;
 }
 
-__global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
+// Boundary cell tid of one scenario (tid < p.ncells).
+__device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= p.ncells) return;
     // enumerate the boundary cell set: left band, right band, then bottom/top bands between them
     int x, y;
     {
@@ -832,7 +851,8 @@ __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
             const float *tb = S.tab + (size_t)strip_index(p, sd, ox, oy) * 5;
             float e = 0.f, hu = 0.f, hv = 0.f;
             if (tb[4] != 0.f) {
-                const float arg = S.omega * p.t - tb[0] - tb[1] + 0.f;
+                const float t = p.clock ? p.clock[0] : p.t;
+                const float arg = S.omega * t - tb[0] - tb[1] + 0.f;
                 e = S.amp * sinf(arg);
                 hu = tb[2] * e;
                 hv = tb[3] * e;
@@ -841,7 +861,12 @@ __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
             break;
         }
         case kUTS:
-            v.w = S.uts_eta + S.sl; v.hu = S.uts_hu; v.hv = S.uts_hv;
+            if (p.clock) {
+                const float *c = p.clock + 1 + 3 * sd;
+                v.w = c[0] + S.sl; v.hu = c[1]; v.hv = c[2];
+            } else {
+                v.w = S.uts_eta + S.sl; v.hu = S.uts_hu; v.hv = S.uts_hv;
+            }
             break;
         default: {   // irregular
             const float *q = S.irr + (size_t)strip_index(p, sd, ox, oy) * 3;
@@ -853,6 +878,102 @@ __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
     p.w[(size_t)y * p.P + x] = v.w;
     p.hu[(size_t)y * p.P + x] = v.hu;
     p.hvT[(size_t)x * p.PT + y] = v.hv;
+}
+
+__global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < p.ncells) boundary_cell(p, tid);
+}
+
+// ------------------------------------------------------------------ diagnostics
+
+// GetStats, compute.hlsl:1319-1378: per-cell values in FP32 exactly as the shader forms
+// them; the five sums are accumulated in FP64, the four maxima in FP32 (order-free).
+// Block partials go to p.partial; stats_final reduces them in a fixed order, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void stats_partial(const StatsParams p)
+{
+    __shared__ float hvs[kStatsTileY][kStatsTileX + 1];
+    __shared__ double red[4][kStatsValues];
+    const int tid = threadIdx.x;
+    const int tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
+    const int x0 = 2 + tx * kStatsTileX, y0 = 2 + ty * kStatsTileY;
+    // hv tile through LDS: hvT is column-major, lanes walk y
+    for (int k = tid; k < kStatsTileX * kStatsTileY; k += 256) {
+        const int xl = k / kStatsTileY, yl = k % kStatsTileY;
+        const int x = x0 + xl, y = y0 + yl;
+        hvs[yl][xl] = (x < 2 + p.ncx && y < 2 + p.ncy) ? p.hvT[(size_t)x * p.PT + y] : 0.f;
+    }
+    __syncthreads();
+    double sh = 0, sb = 0, shu = 0, shv = 0, sk = 0;
+    float mu = 0.f, mh = 0.f, mc = 0.f, mf = 0.f;
+    const int xl = tid % kStatsTileX;
+    for (int yl = tid / kStatsTileX; yl < kStatsTileY; yl += 256 / kStatsTileX) {
+        const int x = x0 + xl, y = y0 + yl;
+        if (x >= 2 + p.ncx || y >= 2 + p.ncy) continue;
+        const size_t i = (size_t)y * p.P + x;
+        const float w = p.w[i], hu = p.hu[i], hv = hvs[yl][xl];
+        const float B = p.BA[i];
+        const float h = fmaxf(0.f, w - B);
+        const float c = sqrtf(p.g * h);
+        const float dbh = divide_by_h(h, p.epsilon);
+        const float u = dbh * hu, v = dbh * hv;
+        sh += (double)h;
+        sb += (double)(h * (B + 0.5f * h));
+        shu += (double)hu;
+        shv += (double)hv;
+        sk += (double)(hu * u + hv * v);
+        const float u2v2 = u * u + v * v;
+        mu = fmaxf(mu, u2v2);
+        mh = fmaxf(mh, h);
+        mc = fmaxf(mc, (fabsf(u) + c) * p.one_over_dx);
+        mc = fmaxf(mc, (fabsf(v) + c) * p.one_over_dy);
+        mf = fmaxf(mf, u2v2 * dbh);
+    }
+    const double vals[kStatsValues] = {wave_sum(sh), wave_sum(sb), wave_sum(shu), wave_sum(shv), wave_sum(sk),
+                                       (double)wave_max(mu), (double)wave_max(mh), (double)wave_max(mc),
+                                       (double)wave_max(mf)};
+    const int wv = tid >> 6, lane = tid & 63;
+    if (lane == 0)
+        for (int v = 0; v < kStatsValues; ++v) red[wv][v] = vals[v];
+    __syncthreads();
+    if (tid < kStatsValues) {
+        double r = red[0][tid];
+        for (int k = 1; k < 4; ++k) r = tid < 5 ? r + red[k][tid] : fmax(r, red[k][tid]);
+        p.partial[(size_t)blockIdx.x * kStatsValues + tid] = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void stats_final(const StatsParams p)
+{
+    const int lane = threadIdx.x;
+    double acc[kStatsValues];
+    for (int v = 0; v < kStatsValues; ++v) acc[v] = 0.0;
+    for (int b = lane; b < p.nblocks; b += 64)
+        for (int v = 0; v < kStatsValues; ++v) {
+            const double x = p.partial[(size_t)b * kStatsValues + v];
+            acc[v] = v < 5 ? acc[v] + x : fmax(acc[v], x);
+        }
+    for (int v = 0; v < kStatsValues; ++v) {
+        double r = acc[v];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double q = __shfl_xor(r, o, 64);
+            r = v < 5 ? r + q : fmax(r, q);
+        }
+        if (lane == 0) p.out[v] = r;
+    }
 }
 
 // ------------------------------------------------------------------ utilities
@@ -985,6 +1106,13 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st)
 {
     hipLaunchKernelGGL(static_fields, dim3((p.nx + 127) / 128, p.ny), dim3(128), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(const StatsParams &p, hipStream_t st)
+{
+    if (p.nblocks > 0) hipLaunchKernelGGL(stats_partial, dim3(p.nblocks), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(stats_final, dim3(1), dim3(64), 0, st, p);
     return hipGetLastError();
 }
 

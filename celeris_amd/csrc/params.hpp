@@ -85,10 +85,16 @@ struct IrregularSide {
     float *out;                          // [(strip * nlines + line) * 3]
 };
 
+// Step clock: per-step values that change every step while the launch pattern does not.
+// A captured multi-step block reads them from a device table, one entry per step:
+// [0] = t^n, [1 + 3 side + {0, 1, 2}] = the uniform-time-series (eta, hu, hv) of a side.
+constexpr int kClockStride = 16;
+
 struct IrregularAll {
     IrregularSide side[4];
     int count[4];      // width * nlines per side (0 if inactive)
     float t;
+    const float *clock;   // if set, t = clock[0]
 };
 
 struct SideParams {
@@ -107,8 +113,21 @@ struct BoundaryParams {
     int ncells;
     int mirror_pass;                      // 1: cells whose chain passes a solid side; 0: all others
     float t;
+    const float *clock;                   // if set, t and the time-series values come from it
     float *w, *hu, *hvT;
     SideParams side[4];
+};
+
+// GetStats (compute.hlsl:1319-1378): cells [2, 2 + ncx) x [2, 2 + ncy), tiles of
+// kStatsTileX x kStatsTileY cells per block, nine partials per block.
+constexpr int kStatsTileX = 64, kStatsTileY = 16, kStatsValues = 9;
+struct StatsParams {
+    int ncx, ncy, P, PT;
+    int tiles_x, nblocks;
+    const float *w, *hu, *hvT, *BA;
+    float g, one_over_dx, one_over_dy, epsilon;
+    double *partial;   // [block * kStatsValues + v]
+    double *out;       // kStatsValues
 };
 
 }  // namespace cel

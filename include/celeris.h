@@ -128,6 +128,15 @@ int cel_engine_set_dt(cel_engine *e, float dt);
  * on the engine's HIP stream. */
 int cel_engine_step(cel_engine *e, int nsteps);
 
+/* Multi-step capture: with steps > 0 (at most 1024), cel_engine_step runs whole blocks
+ * of that many steps as one hipGraph launch each (captured once per starting state of
+ * the ping-pong buffers, history ring and scheme counters, then replayed with that
+ * block's step clock -- t^n and time-series values -- copied in beforehand), and the
+ * remainder step by step.  Results are identical to step-by-step execution.  0 turns
+ * capture off.  Not used while profiling is on.  No counterpart in the reference, whose
+ * frame loop issues every pass from the host. */
+int cel_engine_set_graph_steps(cel_engine *e, int steps);
+
 /* Blocks until queued steps finish. */
 int cel_engine_sync(cel_engine *e);
 
@@ -153,6 +162,39 @@ int cel_engine_kernel_time(cel_engine *e, const char *name, float *ms, int *laun
 
 /* Device bytes held by the engine. */
 int cel_engine_device_bytes(cel_engine *e, size_t *bytes);
+
+/* GetStats + resetTimestep's reductions (compute.hlsl:1319-1378, engine.cpp:1633-1696).
+ * Per-cell values are the shader's FP32 ones; sums are reduced in FP64 on the device
+ * over the cells the reference's 4x4 blocks cover (nx/4 by ny/4 blocks from cell
+ * (2, 2)).  Derived values use density 1000 kg/m^3 and the cell area dx*dy, as the
+ * reference does, but in FP64. */
+typedef struct {
+    double sum_h, sum_bh_h2, sum_hu, sum_hv, sum_hu2v2;      /* raw sums */
+    double max_u2v2, max_h, max_cfl_rate, max_u2v2_h;        /* raw maxima */
+    double mass, x_momentum, y_momentum;
+    double kinetic_energy, potential_energy, total_energy;
+    double max_speed, max_depth, max_froude;
+    double cfl_number;           /* max_cfl_rate * dt */
+    float timestep;              /* dt of the next step */
+    int cells;                   /* cells reduced */
+} cel_stats;
+
+/* The settings resetTimestep reads for the adaptive dt (settings.hpp:333-337 and the
+ * "target cfl" slider, settings.cpp:120 / terrain_heightfield.cpp:207). */
+typedef struct {
+    float ema_alpha;             /* exponential_moving_average_alpha (reference 0.5) */
+    float min_timestep_ratio;    /* min_timestep_ratio (reference 0.05) */
+    float target_cfl;            /* "target cfl" until the first adaptive call sets it */
+} cel_adaptive;
+
+/* GetStats of the current state (synchronises). */
+int cel_engine_stats(cel_engine *e, cel_stats *out);
+
+/* resetTimestep (engine.cpp:1536-1756) without the GUI parts: GetStats, then the dt of
+ * the next step -- adaptive (max(target/rate, timestep*min_ratio), moving average when
+ * growing) while the scheme is CEL_PREDICTOR_ADAPTIVE, else the uniform dt (the
+ * settings' timestep, or the last cel_engine_set_dt).  out may be NULL. */
+int cel_engine_reset_timestep(cel_engine *e, const cel_adaptive *a, cel_stats *out);
 
 /* Level-0 tridiagonal coefficients as the engine uses them (engine.cpp:2638-2694,
  * 2781-2826): three planes A, B, C for x and for y, computed on the device. */

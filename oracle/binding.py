@@ -55,6 +55,9 @@ def lib():
             getattr(L, f"orc_{p}_scheme").argtypes = [ctypes.c_void_p]
             getattr(L, f"orc_{p}_coefficients").argtypes = [ctypes.c_void_p, R, R]
             getattr(L, f"orc_{p}_cr_solve").argtypes = [ctypes.c_int, R, R, R, R, R]
+            getattr(L, f"orc_{p}_stats").argtypes = [ctypes.c_void_p, _D]
+            getattr(L, f"orc_{p}_update_timestep").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4 + [_D]
+            getattr(L, f"orc_{p}_update_timestep").restype = ctypes.c_float
         L.orc_bottom_from_corners.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
         L.orc_f32_minmod.restype = ctypes.c_float
         L.orc_f32_minmod.argtypes = [ctypes.c_float] * 3
@@ -92,6 +95,7 @@ class Oracle:
         L = lib()
         self.p = precision
         self.nx, self.ny = s.nx, s.ny
+        self._uniform_dt = s.timestep
         cfg = OrcConfig()
         for name, _ in OrcConfig._fields_[:-1]:
             setattr(cfg, name, getattr(s, name))
@@ -118,6 +122,7 @@ class Oracle:
         getattr(lib(), f"orc_{self.p}_step")(self.h, n)
 
     def set_dt(self, dt):
+        self._uniform_dt = dt
         getattr(lib(), f"orc_{self.p}_set_dt")(self.h, dt)
 
     def state(self):
@@ -134,6 +139,19 @@ class Oracle:
     @property
     def scheme(self):
         return getattr(lib(), f"orc_{self.p}_scheme")(self.h)
+
+    def stats(self):
+        """GetStats sums/maxima (9 doubles, order as in oracle_impl.inc)."""
+        out = np.zeros(9, np.float64)
+        getattr(lib(), f"orc_{self.p}_stats")(self.h, dptr(out))
+        return out
+
+    def update_timestep(self, ema_alpha=0.5, min_ratio=0.05, target_cfl=0.0, uniform_dt=None):
+        """resetTimestep's dt choice; returns (dt, stats9)."""
+        out = np.zeros(9, np.float64)
+        u = self._uniform_dt if uniform_dt is None else uniform_dt
+        dt = getattr(lib(), f"orc_{self.p}_update_timestep")(self.h, ema_alpha, min_ratio, target_cfl, u, dptr(out))
+        return dt, out
 
     def coefficients(self):
         dt = np.float32 if self.p == "f32" else np.float64

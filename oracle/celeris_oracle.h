@@ -76,6 +76,13 @@ int orc_f32_scheme(const orc_sim_f32 *s);
 /* Level-0 tridiagonal coefficients (A, B, C planes) for x and y, engine.cpp:2638-2694, 2781-2826. */
 void orc_f32_coefficients(const orc_sim_f32 *s, float *abc_x3, float *abc_y3);
 
+/* GetStats (compute.hlsl:1319-1378) of the current state: sums in double over the
+ * reference's 4x4-block domain; see oracle_impl.inc for the nine values. */
+void orc_f32_stats(const orc_sim_f32 *s, double *out9);
+/* resetTimestep's dt choice (engine.cpp:1698-1729); returns the new dt. */
+float orc_f32_update_timestep(orc_sim_f32 *s, float ema_alpha, float min_ratio, float target_cfl, float uniform_dt,
+                              double *stats9);
+
 orc_sim_f64 *orc_f64_create(const orc_config *cfg, const float *bottom, const float *state);
 void orc_f64_destroy(orc_sim_f64 *s);
 void orc_f64_step(orc_sim_f64 *s, int nsteps);
@@ -84,6 +91,9 @@ void orc_f64_get_state(const orc_sim_f64 *s, double *state3);
 double orc_f64_time(const orc_sim_f64 *s);
 int orc_f64_scheme(const orc_sim_f64 *s);
 void orc_f64_coefficients(const orc_sim_f64 *s, double *abc_x3, double *abc_y3);
+void orc_f64_stats(const orc_sim_f64 *s, double *out9);
+float orc_f64_update_timestep(orc_sim_f64 *s, float ema_alpha, float min_ratio, float target_cfl, float uniform_dt,
+                              double *stats9);
 
 /* Terrain preprocessing, terrain_heightfield.cpp:244-298: corner heights
  * z[(ny+5)*(nx+5)] -> bottom planes BY, BX, BA. */

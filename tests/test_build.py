@@ -1,0 +1,38 @@
+"""Build checks of the HIP kernels (no GPU needed): every kernel compiles for gfx950
+without private-memory (scratch) traffic, and the explicit stencil keeps its occupancy.
+A kernel that spills or copies its parameter block to scratch loses a large part of its
+throughput without changing any result, so only a build check catches it."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from celeris_amd import build as B
+
+
+@pytest.fixture(scope="module")
+def resource_report(tmp_path_factory):
+    out = tmp_path_factory.mktemp("res") / "k.o"
+    cmd = [os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")] + [f for f in B.FLAGS if f not in ("-shared", "-fPIC")]
+    cmd += ["--cuda-device-only", "-c", os.path.join(B.CSRC, "kernels.hip"), "-o", str(out),
+            "-Rpass-analysis=kernel-resource-usage"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    kernels = {}
+    for block in r.stderr.split("Function Name: ")[1:]:
+        name = block.split()[0]
+        vals = dict(re.findall(r"(ScratchSize \[bytes/lane\]|VGPRs Spill|Occupancy \[waves/SIMD\]): (\d+)", block))
+        kernels[name] = {k.split()[0]: int(v) for k, v in vals.items()}
+    return kernels
+
+
+def test_no_kernel_uses_scratch(resource_report):
+    assert len(resource_report) >= 10
+    bad = {k: v for k, v in resource_report.items() if v.get("ScratchSize", 0) or v.get("VGPRs", 0)}
+    assert not bad, f"kernels with scratch or spills: {bad}"
+
+
+def test_explicit_step_occupancy(resource_report):
+    occ = [v["Occupancy"] for k, v in resource_report.items() if "explicit_step" in k]
+    assert occ and min(occ) >= 4, occ
