@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("CELERIS_AMD_LIB", os.path.join(HERE, "libceleris_amd.
 SOLID, SPONGE, SINEWAVE, IRREGULARWAVES, UNIFORMTIMESERIES = range(5)
 WEST, EAST, SOUTH, NORTH = range(4)
 EULER, PREDICTOR, CORRECTOR, PREDICTOR_ADAPTIVE = range(4)
+STATE_FP32, STATE_COMPENSATED = range(2)
 STATUS = {0: "CEL_OK", 1: "CEL_EINVAL", 2: "CEL_ENOMEM", 3: "CEL_EDEVICE", 4: "CEL_ENUMERIC", 5: "CEL_ESTATE"}
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -32,7 +33,7 @@ class CelSettings(ctypes.Structure):
                 ("still_water_elevation", ctypes.c_float), ("is_boussinesq", ctypes.c_int),
                 ("epsilon", ctypes.c_float), ("theta", ctypes.c_float), ("friction", ctypes.c_float),
                 ("is_manning", ctypes.c_int), ("correction_steps_num", ctypes.c_int), ("timestep", ctypes.c_float),
-                ("time_scheme", ctypes.c_int), ("gravity", ctypes.c_float)]
+                ("time_scheme", ctypes.c_int), ("gravity", ctypes.c_float), ("state_precision", ctypes.c_int)]
 
 
 class CelSoliton(ctypes.Structure):
@@ -68,6 +69,7 @@ SIGNATURES = {
     "cel_engine_sync": ([ctypes.c_void_p], ctypes.c_int),
     "cel_engine_get_state": ([ctypes.c_void_p, _F], ctypes.c_int),
     "cel_engine_set_state": ([ctypes.c_void_p, _F], ctypes.c_int),
+    "cel_engine_get_state_f64": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cel_engine_time": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cel_engine_scheme": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_engine_timed_steps": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
@@ -117,10 +119,11 @@ def _fptr(a):
 
 
 def settings(nx, ny, width, length, timestep, still_water_elevation=0.0, is_boussinesq=1, epsilon=5e-8, theta=1.3,
-             friction=0.0, is_manning=0, correction_steps_num=0, time_scheme=PREDICTOR, gravity=9.81):
+             friction=0.0, is_manning=0, correction_steps_num=0, time_scheme=PREDICTOR, gravity=9.81,
+             state_precision=STATE_FP32):
     """cel_settings with the reference's defaults (settings.hpp:376-435; g from gui_manager.cpp:107)."""
     return CelSettings(nx, ny, width, length, still_water_elevation, is_boussinesq, epsilon, theta, friction,
-                       is_manning, correction_steps_num, timestep, time_scheme, gravity)
+                       is_manning, correction_steps_num, timestep, time_scheme, gravity, state_precision)
 
 
 class Boundary:
@@ -207,6 +210,12 @@ class ShallowWaterEngine:
     def get_state(self):
         out = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)
         _check(load_library().cel_engine_get_state(self._h, _fptr(out)))
+        return out
+
+    def get_state_f64(self):
+        """hi + lo words in FP64 (compensated engines), else the FP32 state widened."""
+        out = np.zeros((3, self.ny + 4, self.nx + 4), np.float64)
+        _check(load_library().cel_engine_get_state_f64(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
     def set_state(self, state):

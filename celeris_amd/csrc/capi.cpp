@@ -108,6 +108,7 @@ int cel_engine_set_dt(cel_engine *e, float dt) { return guarded([&] { eng(e).res
 int cel_engine_step(cel_engine *e, int nsteps) { return guarded([&] { eng(e).step(nsteps); }); }
 int cel_engine_sync(cel_engine *e) { return guarded([&] { eng(e).sync(); }); }
 int cel_engine_get_state(cel_engine *e, float *state3) { return guarded([&] { eng(e).getState(state3); }); }
+int cel_engine_get_state_f64(cel_engine *e, double *state3) { return guarded([&] { eng(e).getStateF64(state3); }); }
 int cel_engine_set_state(cel_engine *e, const float *state3) { return guarded([&] { eng(e).setState(state3); }); }
 
 int cel_engine_time(cel_engine *e, double *t)

@@ -58,6 +58,11 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
     if (s.time_scheme != CEL_PREDICTOR && s.time_scheme != CEL_PREDICTOR_ADAPTIVE)
         throw Error(CEL_EINVAL, "time_scheme must be CEL_PREDICTOR or CEL_PREDICTOR_ADAPTIVE");
     if (s.correction_steps_num < 0) throw Error(CEL_EINVAL, "correction_steps_num must be >= 0");
+    if (s.state_precision != CEL_STATE_FP32 && s.state_precision != CEL_STATE_COMPENSATED)
+        throw Error(CEL_EINVAL, "state_precision must be CEL_STATE_FP32 or CEL_STATE_COMPENSATED");
+    comp_ = s.state_precision == CEL_STATE_COMPENSATED;
+    if (comp_ && s.correction_steps_num > 0)
+        throw Error(CEL_EINVAL, "the compensated state update does not take corrector steps");
     nx_ = s.nx; ny_ = s.ny; X_ = nx_ + 4; Y_ = ny_ + 4;
     P_ = (int)round_up((size_t)X_ + 16, 64);
     PT_ = (int)round_up((size_t)Y_ + 16, 64);
@@ -101,8 +106,9 @@ void ShallowWaterEngine::allocate()
 {
     const size_t plane = (size_t)P_ * Y_ * sizeof(float);
     const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
-    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6;   // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; static
-    const int ncol = 2 + 1 + 1 + 2;                           // hvT x2; DyT; ChvT; KyT, QyT
+    // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; static; compensated w_lo, hu_lo x2
+    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6 + (comp_ ? 4 : 0);
+    const int ncol = 2 + 1 + 1 + 2 + (comp_ ? 2 : 0);         // hvT x2; DyT; ChvT; KyT, QyT; hvT_lo x2
     arena_bytes_ = plane * nrow + tplane * ncol;
     void *mem = nullptr;
     check(hipMalloc(&mem, arena_bytes_), "hipMalloc(state)");
@@ -120,6 +126,10 @@ void ShallowWaterEngine::allocate()
     Cw_ = rp(); Chu_ = rp();
     Kx_ = rp(); Qx_ = rp(); Ky_ = rp(); Qy_ = rp(); DDx_ = rp(); DDy_ = rp();
     hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); KyT_ = cp(); QyT_ = cp();
+    if (comp_) {
+        w_lo_[0] = rp(); w_lo_[1] = rp(); hu_lo_[0] = rp(); hu_lo_[1] = rp();
+        hvT_lo_[0] = cp(); hvT_lo_[1] = cp();
+    }
     const int tiles = ((4 * (nx_ / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (ny_ / 4) + kStatsTileY - 1) / kStatsTileY);
     stats_blocks_ = tiles;
     const size_t sbytes = ((size_t)tiles + 1) * kStatsValues * sizeof(double);
@@ -180,6 +190,14 @@ void ShallowWaterEngine::setState(const float *state3)
     const size_t plane = (size_t)P_ * Y_ * sizeof(float);
     for (auto &g : G_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
     for (auto &g : FG_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
+    if (comp_) {
+        const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
+        for (int k = 0; k < 2; ++k) {
+            check(hipMemsetAsync(w_lo_[k], 0, plane, stream_), "zero low words");
+            check(hipMemsetAsync(hu_lo_[k], 0, plane, stream_), "zero low words");
+            check(hipMemsetAsync(hvT_lo_[k], 0, tplane, stream_), "zero low words");
+        }
+    }
     check(hipStreamSynchronize(stream_), "upload sync");
     resetScheme();
 }
@@ -199,6 +217,27 @@ void ShallowWaterEngine::getState(float *state3)
     float *hv = state3 + 2 * N;
     for (int y = 0; y < Y_; ++y)
         for (int x = 0; x < X_; ++x) hv[(size_t)y * X_ + x] = t[(size_t)x * Y_ + y];
+}
+
+void ShallowWaterEngine::getStateF64(double *state3)
+{
+    if (!state3) throw Error(CEL_EINVAL, "null state");
+    const size_t N = (size_t)X_ * Y_;
+    std::vector<float> hi(3 * N), lo(3 * N, 0.f);
+    getState(hi.data());
+    if (comp_) {
+        check(hipMemcpy2DAsync(lo.data(), (size_t)X_ * 4, w_lo_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
+                               hipMemcpyDeviceToHost, stream_), "download w_lo");
+        check(hipMemcpy2DAsync(lo.data() + N, (size_t)X_ * 4, hu_lo_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
+                               hipMemcpyDeviceToHost, stream_), "download hu_lo");
+        std::vector<float> t(N);
+        check(hipMemcpy2DAsync(t.data(), (size_t)Y_ * 4, hvT_lo_[cur_], (size_t)PT_ * 4, (size_t)Y_ * 4, X_,
+                               hipMemcpyDeviceToHost, stream_), "download hv_lo");
+        check(hipStreamSynchronize(stream_), "download sync");
+        for (int y = 0; y < Y_; ++y)
+            for (int x = 0; x < X_; ++x) lo[2 * N + (size_t)y * X_ + x] = t[(size_t)x * Y_ + y];
+    }
+    for (size_t i = 0; i < 3 * N; ++i) state3[i] = (double)hi[i] + (double)lo[i];
 }
 
 void ShallowWaterEngine::freeSideTables(int side)
@@ -465,6 +504,7 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     for (int k = 0; k < 2; ++k) { p.FGo[k] = FG_[ring_old_][k]; p.FGoo[k] = FG_[ring_oo_][k]; p.FGs[k] = FG_[ring_scr_][k]; p.FGp[k] = FGp_[k]; }
     p.Cw = Cw_; p.Chu = Chu_; p.ChvT = ChvT_;
     p.w_new = w_[n]; p.Dx = Dx_; p.DyT = DyT_;
+    p.w_lo = w_lo_[c]; p.w_lo_new = w_lo_[n];
     // fillConstantBuffers, engine.cpp:4099-4136
     const float g = s_.gravity;
     p.two_theta = 2 * s_.theta;
@@ -524,6 +564,9 @@ void ShallowWaterEngine::fillLines(LineParams &p) const
     p.DyT = DyT_; p.KyT = KyT_; p.QyT = QyT_;
     const int n = 1 - cur_;
     p.hu_out = hu_[n]; p.hvT_out = hvT_[n];
+    const int c = cur_;
+    p.hu_prev = hu_[c]; p.hu_prev_lo = hu_lo_[c]; p.hu_lo_out = hu_lo_[n];
+    p.hvT_prev = hvT_[c]; p.hvT_prev_lo = hvT_lo_[c]; p.hvT_lo_out = hvT_lo_[n];
     p.Lx = Lx_; p.Ly = Ly_;
 }
 
@@ -586,6 +629,7 @@ void ShallowWaterEngine::planStep(StepLaunch &L)
     bp.ncells = (bp.band_l + bp.band_r) * Y_ + (X_ - bp.band_l - bp.band_r) * (bp.band_b + bp.band_t);
     bp.t = total_time_;
     bp.w = w_[n]; bp.hu = hu_[n]; bp.hvT = hvT_[n];
+    bp.w_lo = w_lo_[n]; bp.hu_lo = hu_lo_[n]; bp.hvT_lo = hvT_lo_[n];
     L.clock[0] = total_time_;
     for (int s = 0; s < 4; ++s) {
         SideParams &S = bp.side[s];
@@ -630,7 +674,7 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, boo
         for (auto &e : pe) e = nextEvent();
         check(hipEventRecord(pe[0], stream_), "event");
     }
-    check(launch_explicit(L.scheme, L.ep, stream_), "explicit_step");
+    check(launch_explicit(L.scheme, comp_, L.ep, stream_), "explicit_step");
     if (profile) check(hipEventRecord(pe[1], stream_), "event");
     for (int i = 0; i < L.ncopy; ++i)
         check(hipMemcpyAsync(L.copy[i].dst, L.copy[i].src, L.copy[i].bytes, hipMemcpyDeviceToDevice, stream_), "copy");
@@ -639,7 +683,7 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, boo
     irr.clock = dclock;
     bp.clock = dclock;
     if (profile) check(hipEventRecord(pe[2], stream_), "event");
-    check(launch_solve(L.lp, irr, stream_), "solve_lines");
+    check(launch_solve(comp_, L.lp, irr, stream_), "solve_lines");
     if (profile) check(hipEventRecord(pe[3], stream_), "event");
     bp.mirror_pass = 1;
     check(launch_boundary(bp, stream_), "boundary_apply");

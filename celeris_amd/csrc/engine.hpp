@@ -24,8 +24,8 @@ struct Error : std::runtime_error {
     Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
 };
 
-hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st);
-hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_t st);
+hipError_t launch_explicit(int scheme, bool comp, const ExplicitParams &p, hipStream_t st);
+hipError_t launch_solve(bool comp, const LineParams &p, const IrregularAll &irr, hipStream_t st);
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
 hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st);
@@ -76,6 +76,7 @@ public:
     void step(int n);
     void sync();
     void getState(float *state3);
+    void getStateF64(double *state3);
     void setState(const float *state3);
     double time() const { return total_time_; }
     int scheme() const { return time_scheme_now_; }
@@ -133,6 +134,8 @@ private:
     char *arena_ = nullptr;
     size_t arena_bytes_ = 0, table_bytes_ = 0;
     float *w_[2], *hu_[2], *hvT_[2];
+    bool comp_ = false;                                       // compensated two-float state
+    float *w_lo_[2] = {}, *hu_lo_[2] = {}, *hvT_lo_[2] = {};
     float *BY_, *BX_, *BA_;
     float *Kx_, *Qx_, *Ky_, *Qy_, *DDx_, *DDy_, *KyT_, *QyT_;   // static per-cell fields
     float *Dx_, *DyT_;

@@ -92,6 +92,14 @@ __device__ __forceinline__ float num_flux(float ap, float am, float Fp, float Fm
     return den != 0.f ? q : 0.f;
 }
 
+// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no contraction).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
 // Element access through a 32-bit byte offset from a uniform base: maps to the
 // scalar-base + vector-offset global addressing mode (planes are < 4 GiB).
 __device__ __forceinline__ float ld32(const float *base, unsigned idx)
@@ -210,7 +218,7 @@ struct ExplicitSmem {
 
 // One wave computes interior columns [x0, x0 + 60) of rows [ys, ys + R) of one scenario.
 // SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
-template <int SCHEME, bool CS>
+template <int SCHEME, bool COMP, bool CS>
 __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS> &sm)
 {
     constexpr int R = kBandRows;
@@ -416,21 +424,30 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
 
             const float dt = p.dt;
             float wOut, huOut, hvOut;
+            // increments of w and the line right-hand sides without the level-0 A, B, C
+            // terms: the compensated update solves for the increment of hu and hv
+            float dw, dhu, dhv;
             if (SCHEME == 0) {
-                wOut = wc + dt * dr;
-                huOut = Ax * hul + Bx * huc + Cx * hur + dt * dg;
-                hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt * db;
+                dw = dt * dr;
+                dhu = dt * dg;
+                dhv = dt * db;
+                wOut = wc + dw;
+                huOut = Ax * hul + Bx * huc + Cx * hur + dhu;
+                hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dhv;
             } else {
                 const float o0 = cin.o0, o1 = cin.o1, o2 = cin.o2;
                 const float oo0 = cin.oo0, oo1 = cin.oo1, oo2 = cin.oo2;
                 const float Fo = cin.Fo, Go_ = cin.Go;
                 const float Foo = cin.Foo, Goo_ = cin.Goo;
                 if (SCHEME == 1) {
-                    wOut = wc + dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
-                    huOut = Ax * hul + Bx * huc + Cx * hur + dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1) +
-                            (2.f * F_star - 3.f * Fo + Foo);
-                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt / 12.f * (23.f * db - 16.f * o2 + 5.f * oo2) +
-                            (2.f * G_star - 3.f * Go_ + Goo_);
+                    dw = dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
+                    const float u1 = dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1), u2 = 2.f * F_star - 3.f * Fo + Foo;
+                    const float v1 = dt / 12.f * (23.f * db - 16.f * o2 + 5.f * oo2), v2 = 2.f * G_star - 3.f * Go_ + Goo_;
+                    wOut = wc + dw;
+                    huOut = Ax * hul + Bx * huc + Cx * hur + u1 + u2;
+                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + v1 + v2;
+                    dhu = u1 + u2;
+                    dhv = v1 + v2;
                 } else if (SCHEME == 3) {
                     const float *c = p.der;
                     const float dF0 = (c[0] * F_star) + (c[1] * Fo) + (c[2] * Foo);
@@ -440,11 +457,14 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                     const float dG1 = (c[3] * G_star) + (c[4] * Go_) + (c[5] * Goo_);
                     const float dG2 = (c[6] * G_star) + (c[7] * Go_) + (c[8] * Goo_);
                     const float a0 = p.ab[0], a1 = p.ab[1], a2 = p.ab[2];
-                    wOut = wc + dt / 6.f * (a0 * dr + a1 * o0 + a2 * oo0);
-                    huOut = Ax * hul + Bx * huc + Cx * hur + dt / 6.f * (a0 * dg + a1 * o1 + a2 * oo1) +
-                            dt / 6.f * (a0 * dF0 + a1 * dF1 + a2 * dF2);
-                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2) +
-                            dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
+                    dw = dt / 6.f * (a0 * dr + a1 * o0 + a2 * oo0);
+                    const float u1 = dt / 6.f * (a0 * dg + a1 * o1 + a2 * oo1), u2 = dt / 6.f * (a0 * dF0 + a1 * dF1 + a2 * dF2);
+                    const float v1 = dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2), v2 = dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
+                    wOut = wc + dw;
+                    huOut = Ax * hul + Bx * huc + Cx * hur + u1 + u2;
+                    hvOut = Ay * hv_down + By * hvc + Cy * hv_up + v1 + v2;
+                    dhu = u1 + u2;
+                    dhv = v1 + v2;
                 } else {
                     const float cw = ld32(p.Cw, i);
                     const float chu = ld32(p.Chu, i), chul = ld32(p.Chu, i - 1u), chur = ld32(p.Chu, i + 1u);
@@ -458,7 +478,16 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                             F_star - Fp;
                     hvOut = Ay * chvd + By * chv + Cy * chvu + dt / 24.f * (9.f * db + 19.f * pr2 - 5.f * o2 + 1.f * oo2) +
                             G_star - Gp_;
+                    dw = dhu = dhv = 0.f;   // the compensated update does not take corrector steps
                 }
+            }
+            if (COMP) {
+                // (w_hi, w_lo) += dw as a two-float sum; the lines solve for the increments
+                float lo;
+                two_sum(wc, dw + ld32(p.w_lo, i), wOut, lo);
+                st32(p.w_lo_new, i, lo);
+                huOut = dhu;
+                hvOut = dhv;
             }
             st32(p.w_new, i, wOut);
             st32(p.Dx, i - 2u, huOut);
@@ -491,7 +520,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
 }
 
-template <int SCHEME>
+template <int SCHEME, bool COMP>
 __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 {
     __shared__ ExplicitSmem<SCHEME == 2> sm;
@@ -501,7 +530,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     const int per_xcd = (ntiles + 7) / 8;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
-    explicit_tile<SCHEME>(p, tile, sm);
+    explicit_tile<SCHEME, COMP>(p, tile, sm);
 }
 
 // ------------------------------------------------------------------ line solves
@@ -617,8 +646,11 @@ __device__ __forceinline__ void load_chunk(const float *src, int base, int n, fl
 // Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
 // D, K, Q point at element 0 of the line (16-byte aligned); out at element 0 of the
 // ghost-inclusive output line + 2 (8-byte aligned).
-template <int C, int W>
-__device__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out, LineSmem<C, W> &sm)
+// COMP: the solution is an increment; out/out_lo receive the two-float sum of the previous
+// state (prev, prev_lo, same layout as out) and the increment.
+template <int C, int W, bool COMP>
+__device__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out, LineSmem<C, W> &sm,
+                              const float *prev, const float *prev_lo, float *out_lo)
 {
     constexpr int NT = 64 * W;
     constexpr int K0 = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : C == 16 ? 4 : 5;
@@ -703,7 +735,16 @@ __device__ void cr_line_block(int n, const float *D, const float *K, const float
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
-    if (C % 2 == 0 && base + C <= n) {
+    if (COMP) {
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+            if (base + i < n) {
+                float hi, lo;
+                two_sum(prev[base + i], q[i].d + prev_lo[base + i], hi, lo);
+                out[base + i] = hi;
+                out_lo[base + i] = lo;
+            }
+    } else if (C % 2 == 0 && base + C <= n) {
 #pragma unroll
         for (int i = 0; i < C; i += 2) *reinterpret_cast<float2 *>(out + base + i) = make_float2(q[i].d, q[i + 1].d);
     } else {
@@ -716,7 +757,7 @@ __device__ void cr_line_block(int n, const float *D, const float *K, const float
 // One block: an x-line, a y-line, or W irregular-wave lines.  (The body stays in the
 // kernel: routing the by-value parameter blocks through a device function makes the
 // compiler copy them to scratch.)
-template <int C, int W>
+template <int C, int W, bool COMP>
 __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, const IrregularAll irr)
 {
     __shared__ LineSmem<C, W> sm;
@@ -740,10 +781,14 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     if (xdir) {
         const size_t o = (size_t)line * p.P;
-        cr_line_block<C, W>(p.nx, p.Dx + o, p.Kx + o, p.Qx + o, p.hu_out + o + 2, sm);
+        cr_line_block<C, W, COMP>(p.nx, p.Dx + o, p.Kx + o, p.Qx + o, p.hu_out + o + 2, sm,
+                                  COMP ? p.hu_prev + o + 2 : nullptr, COMP ? p.hu_prev_lo + o + 2 : nullptr,
+                                  COMP ? p.hu_lo_out + o + 2 : nullptr);
     } else {
         const size_t o = (size_t)line * p.PT;
-        cr_line_block<C, W>(p.ny, p.DyT + o, p.KyT + o, p.QyT + o, p.hvT_out + o + 2, sm);
+        cr_line_block<C, W, COMP>(p.ny, p.DyT + o, p.KyT + o, p.QyT + o, p.hvT_out + o + 2, sm,
+                                  COMP ? p.hvT_prev + o + 2 : nullptr, COMP ? p.hvT_prev_lo + o + 2 : nullptr,
+                                  COMP ? p.hvT_lo_out + o + 2 : nullptr);
     }
 }
 
@@ -878,6 +923,11 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid)
     p.w[(size_t)y * p.P + x] = v.w;
     p.hu[(size_t)y * p.P + x] = v.hu;
     p.hvT[(size_t)x * p.PT + y] = v.hv;
+    if (p.w_lo) {   // compensated state: boundary values are set, not accumulated
+        p.w_lo[(size_t)y * p.P + x] = 0.f;
+        p.hu_lo[(size_t)y * p.P + x] = 0.f;
+        p.hvT_lo[(size_t)x * p.PT + y] = 0.f;
+    }
 }
 
 __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
@@ -1046,10 +1096,6 @@ __global__ void export_coefficients(const StaticParams p, float *ax, float *ay)
     ay[i] = A; ay[N + i] = B; ay[2 * N + i] = C;
 }
 
-template __global__ void explicit_step<0>(const ExplicitParams);
-template __global__ void explicit_step<1>(const ExplicitParams);
-template __global__ void explicit_step<2>(const ExplicitParams);
-template __global__ void explicit_step<3>(const ExplicitParams);
 
 }  // namespace cel
 
@@ -1057,19 +1103,30 @@ template __global__ void explicit_step<3>(const ExplicitParams);
 
 namespace cel {
 
-hipError_t launch_explicit(int scheme, const ExplicitParams &p, hipStream_t st)
+hipError_t launch_explicit(int scheme, bool comp, const ExplicitParams &p, hipStream_t st)
 {
     const dim3 grid(8 * ((p.nstrips * p.nbands + 7) / 8)), block(64);
-    switch (scheme) {
-    case 0: hipLaunchKernelGGL(explicit_step<0>, grid, block, 0, st, p); break;
-    case 1: hipLaunchKernelGGL(explicit_step<1>, grid, block, 0, st, p); break;
-    case 2: hipLaunchKernelGGL(explicit_step<2>, grid, block, 0, st, p); break;
-    default: hipLaunchKernelGGL(explicit_step<3>, grid, block, 0, st, p); break;
+#define CEL_EXPLICIT(S, CP) hipLaunchKernelGGL((explicit_step<S, CP>), grid, block, 0, st, p)
+    if (comp) {
+        switch (scheme) {
+        case 0: CEL_EXPLICIT(0, true); break;
+        case 1: CEL_EXPLICIT(1, true); break;
+        case 3: CEL_EXPLICIT(3, true); break;
+        default: return hipErrorInvalidValue;   // no compensated corrector
+        }
+    } else {
+        switch (scheme) {
+        case 0: CEL_EXPLICIT(0, false); break;
+        case 1: CEL_EXPLICIT(1, false); break;
+        case 2: CEL_EXPLICIT(2, false); break;
+        default: CEL_EXPLICIT(3, false); break;
+        }
     }
+#undef CEL_EXPLICIT
     return hipGetLastError();
 }
 
-hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_t st)
+hipError_t launch_solve(bool comp, const LineParams &p, const IrregularAll &irr, hipStream_t st)
 {
     const int nmax = p.nx > p.ny ? p.nx : p.ny;
     if (nmax <= 64 * 8 * 16) {
@@ -1079,7 +1136,8 @@ hipError_t launch_solve(const LineParams &p, const IrregularAll &irr, hipStream_
 #define CEL_SOLVE(C, W)                                                                              \
     do {                                                                                            \
         const dim3 grid(p.nlines_x + p.nlines_y + (irr_lines + (W) - 1) / (W)), block(64 * (W));    \
-        hipLaunchKernelGGL((solve_lines_block<C, W>), grid, block, 0, st, p, irr);                  \
+        if (comp) hipLaunchKernelGGL((solve_lines_block<C, W, true>), grid, block, 0, st, p, irr);   \
+        else hipLaunchKernelGGL((solve_lines_block<C, W, false>), grid, block, 0, st, p, irr);       \
     } while (0)
         if (nmax <= 64) CEL_SOLVE(1, 1);
         else if (nmax <= 128) CEL_SOLVE(2, 1);

@@ -21,6 +21,9 @@ struct ExplicitParams {
     int nstrips, nbands;
     // state at step n
     const float *w, *hu, *hvT;
+    // compensated state: low word of w at n (read) and n+1 (written)
+    const float *w_lo;
+    float *w_lo_new;
     // bottom: BY (north face), BX (east face), BA (centre)
     const float *BY, *BX, *BA;
     // Adams-Bashforth history: dU/dt at n-1 and n-2 (w, hu, hv planes), (F*, G*) at n-1, n-2
@@ -61,6 +64,9 @@ struct LineParams {
     const float *Dx, *Kx, *Qx;      // x-lines: rhs and coefficient fields, [y * P + (x - 2)]
     const float *DyT, *KyT, *QyT;   // y-lines: the same, transposed, [x * PT + (y - 2)]
     float *hu_out, *hvT_out;
+    // compensated state: previous hi/lo words and the new low words (same layouts as out)
+    const float *hu_prev, *hu_prev_lo, *hvT_prev, *hvT_prev_lo;
+    float *hu_lo_out, *hvT_lo_out;
     int Lx, Ly;                     // cyclic-reduction level counts
     // irregular-wave strips, handled by blocks after the line solves
     int n_irr_lines;                // total (side, strip-index, line) triples
@@ -115,6 +121,7 @@ struct BoundaryParams {
     float t;
     const float *clock;                   // if set, t and the time-series values come from it
     float *w, *hu, *hvT;
+    float *w_lo, *hu_lo, *hvT_lo;         // compensated state: low words (zeroed where set)
     SideParams side[4];
 };
 

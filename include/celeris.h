@@ -52,6 +52,15 @@ typedef enum {
     CEL_PREDICTOR_ADAPTIVE = 3
 } cel_time_scheme;
 
+/* State storage and update.  CEL_STATE_FP32 is the reference's arithmetic.
+ * CEL_STATE_COMPENSATED keeps (w, hu, hv) as two-float pairs (hi + lo): the explicit
+ * update adds its increment with an exact two-sum, and the line systems are solved for
+ * the increment of hu and hv, which is then added the same way, so increments far below
+ * the state's FP32 resolution accumulate instead of rounding away.  Every other
+ * operation reads the hi words in FP32.  Not combined with corrector steps.  No
+ * counterpart in the reference. */
+typedef enum { CEL_STATE_FP32 = 0, CEL_STATE_COMPENSATED = 1 } cel_state_precision;
+
 /* Maximum irregular-wave components per side: MAX_NUM_OF_IRREGULAR_WAVES,
  * engine.hpp:216 / compute.hlsl:111-114. */
 #define CEL_MAX_IRREGULAR_WAVES 1000
@@ -87,6 +96,7 @@ typedef struct {
     float timestep;               /* fixed dt, "uniform time step" */
     int time_scheme;              /* CEL_PREDICTOR or CEL_PREDICTOR_ADAPTIVE */
     float gravity;
+    int state_precision;          /* cel_state_precision; 0 = the reference's FP32 */
 } cel_settings;
 
 /* Soliton, settings.hpp:180-192. */
@@ -142,6 +152,8 @@ int cel_engine_sync(cel_engine *e);
 
 /* Copies the current state (w, hu, hv planes) to the host (synchronises). */
 int cel_engine_get_state(cel_engine *e, float *state3);
+/* The state as FP64 planes: hi + lo for a compensated engine, hi otherwise. */
+int cel_engine_get_state_f64(cel_engine *e, double *state3);
 /* Hot start (engine.cpp:2943-2970): replaces the state; the AB history and the
  * scheme counters are reset exactly as after remesh. */
 int cel_engine_set_state(cel_engine *e, const float *state3);
