@@ -77,6 +77,23 @@ SIGNATURES = {
     "cel_engine_kernel_time": ([ctypes.c_void_p, ctypes.c_char_p, _F, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_engine_device_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "cel_engine_coefficients": ([ctypes.c_void_p, _F, _F], ctypes.c_int),
+    "cel_estimate_device_bytes": ([ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary),
+                                   ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "cel_batch_create": ([ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "cel_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "cel_batch_add": ([ctypes.c_void_p, ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
+                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_batch_remove": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_batch_step": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_batch_set_graph_steps": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_batch_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "cel_batch_timed_steps": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
+    "cel_batch_count": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_batch_device_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "cel_batch_get_state": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
+    "cel_batch_get_state_f64": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "cel_batch_stats": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CelStats)], ctypes.c_int),
+    "cel_batch_time": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cel_engine_set_graph_steps": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "cel_engine_stats": ([ctypes.c_void_p, ctypes.POINTER(CelStats)], ctypes.c_int),
     "cel_engine_reset_timestep": ([ctypes.c_void_p, ctypes.POINTER(CelAdaptive), ctypes.POINTER(CelStats)],
@@ -274,3 +291,105 @@ class ShallowWaterEngine:
         ay = np.zeros_like(ax)
         _check(load_library().cel_engine_coefficients(self._h, _fptr(ax), _fptr(ay)))
         return ax, ay
+
+
+def estimate_device_bytes(s, boundaries):
+    """Device bytes an engine with these settings and boundaries holds once created."""
+    cb = (CelBoundary * 4)(*[b.to_c() for b in boundaries])
+    out = ctypes.c_size_t()
+    _check(load_library().cel_estimate_device_bytes(ctypes.byref(s), cb, ctypes.byref(out)))
+    return out.value
+
+
+class Batch:
+    """Independent scenarios advanced together (cel_batch_*): one launch per kernel per step
+    over all scenarios, optionally replayed from captured multi-step graphs."""
+
+    def __init__(self, graph_steps=0):
+        h = ctypes.c_void_p()
+        _check(load_library().cel_batch_create(ctypes.byref(h)))
+        self._h = h
+        self._dims = {}
+        if graph_steps:
+            self.set_graph_steps(graph_steps)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(load_library().cel_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, s, boundaries, bottom, state):
+        cb = (CelBoundary * 4)(*[b.to_c() for b in boundaries])
+        bottom = np.ascontiguousarray(bottom, dtype=np.float32)
+        state = np.ascontiguousarray(state, dtype=np.float32)
+        i = ctypes.c_int()
+        _check(load_library().cel_batch_add(self._h, ctypes.byref(s), cb, _fptr(bottom), _fptr(state), ctypes.byref(i)))
+        self._dims[i.value] = (s.nx, s.ny)
+        return i.value
+
+    def remove(self, sid):
+        _check(load_library().cel_batch_remove(self._h, sid))
+        self._dims.pop(sid, None)
+
+    def step(self, n=1):
+        _check(load_library().cel_batch_step(self._h, n))
+
+    def set_graph_steps(self, k):
+        _check(load_library().cel_batch_set_graph_steps(self._h, k))
+
+    def sync(self):
+        _check(load_library().cel_batch_sync(self._h))
+
+    def timed_steps(self, n):
+        ms = ctypes.c_float()
+        _check(load_library().cel_batch_timed_steps(self._h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def __len__(self):
+        n = ctypes.c_int()
+        _check(load_library().cel_batch_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        _check(load_library().cel_batch_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def _dim(self, sid):
+        if sid not in self._dims:
+            raise CeleriError(1, f"no such scenario {sid}")
+        return self._dims[sid]
+
+    def get_state(self, sid):
+        nx, ny = self._dim(sid)
+        out = np.zeros((3, ny + 4, nx + 4), np.float32)
+        _check(load_library().cel_batch_get_state(self._h, sid, _fptr(out)))
+        return out
+
+    def get_state_f64(self, sid):
+        nx, ny = self._dim(sid)
+        out = np.zeros((3, ny + 4, nx + 4), np.float64)
+        _check(load_library().cel_batch_get_state_f64(self._h, sid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def stats(self, sid):
+        st = CelStats()
+        _check(load_library().cel_batch_stats(self._h, sid, ctypes.byref(st)))
+        return st.as_dict()
+
+    def time(self, sid):
+        t = ctypes.c_double()
+        _check(load_library().cel_batch_time(self._h, sid, ctypes.byref(t)))
+        return t.value

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libceleris_amd.so")
-SOURCES = ["kernels.hip", "engine.cpp", "terrain.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "terrain.cpp", "capi.cpp"]
 
 # FMA contraction off: every product and sum rounds as the reference's shader text is
 # written.  Packed FP32 instructions are disabled for the device.

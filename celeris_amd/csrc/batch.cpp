@@ -1,0 +1,311 @@
+// batch.cpp -- batched scenarios: one launch per kernel per step over all scenarios.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace cel {
+
+namespace {
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+}  // namespace
+
+void Batch::check(hipError_t e, const char *what) const
+{
+    if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? CEL_ENOMEM : CEL_EDEVICE,
+                                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Batch::Batch()
+{
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &e : blob_ev_) check(hipEventCreate(&e), "hipEventCreate");
+    for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+}
+
+Batch::~Batch()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    clearGraphs();
+    scen_.clear();
+    if (d_blob_) (void)hipFree(d_blob_);
+    for (auto &h : h_blob_) if (h) (void)hipHostFree(h);
+    if (d_starts_) (void)hipFree(d_starts_);
+    for (auto &e : blob_ev_) if (e) (void)hipEventDestroy(e);
+    for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Batch::add(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3, const float *state3)
+{
+    if (s.correction_steps_num != 0) throw Error(CEL_EINVAL, "batched scenarios do not take corrector steps");
+    sync();
+    auto e = std::make_unique<ShallowWaterEngine>(s, bnd, bottom3, state3);
+    const int id = next_id_++;
+    scen_[id] = std::move(e);
+    comp_valid_ = false;
+    return id;
+}
+
+void Batch::remove(int id)
+{
+    auto it = scen_.find(id);
+    if (it == scen_.end()) throw Error(CEL_EINVAL, "no such scenario");
+    sync();
+    scen_.erase(it);
+    comp_valid_ = false;
+}
+
+ShallowWaterEngine &Batch::scenario(int id)
+{
+    auto it = scen_.find(id);
+    if (it == scen_.end()) throw Error(CEL_EINVAL, "no such scenario");
+    sync();
+    return *it->second;
+}
+
+void Batch::ids(int *out) const
+{
+    int i = 0;
+    for (const auto &kv : scen_) out[i++] = kv.first;
+}
+
+size_t Batch::deviceBytes() const
+{
+    size_t b = blob_cap_ + starts_cap_ * sizeof(int);
+    for (const auto &kv : scen_) b += kv.second->deviceBytes();
+    return b;
+}
+
+void Batch::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+void Batch::clearGraphs()
+{
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
+void Batch::setGraphSteps(int k)
+{
+    if (k < 0 || k > kMaxGraphSteps) throw Error(CEL_EINVAL, "steps per graph must lie in [0, 1024]");
+    sync();
+    clearGraphs();
+    graph_steps_ = k;
+}
+
+void Batch::ensureBlob(size_t bytes)
+{
+    if (bytes <= blob_cap_) return;
+    sync();
+    clearGraphs();   // graphs hold the old table's address
+    if (d_blob_) (void)hipFree(d_blob_);
+    for (auto &h : h_blob_) if (h) (void)hipHostFree(h);
+    d_blob_ = nullptr;
+    h_blob_[0] = h_blob_[1] = nullptr;
+    const size_t cap = std::max(bytes, blob_cap_ * 2);
+    void *mem = nullptr;
+    check(hipMalloc(&mem, cap), "hipMalloc(batch table)");
+    d_blob_ = static_cast<char *>(mem);
+    for (auto &h : h_blob_) {
+        check(hipHostMalloc(&mem, cap, hipHostMallocDefault), "hipHostMalloc(batch table)");
+        h = static_cast<char *>(mem);
+    }
+    blob_cap_ = cap;
+    blob_used_[0] = blob_used_[1] = false;
+}
+
+// Work counts of each scenario (from its first planned step; they depend on the grid and
+// boundary configuration only), line-length classes, start offsets of every work-list.
+void Batch::buildComposition(const std::vector<StepLaunch> &first, const std::vector<int> &ids)
+{
+    const int ns = (int)ids.size();
+    Composition c;
+    c.ids = ids;
+    std::vector<int> tiles(ns), blocks(ns), cells(ns), cls(ns);
+    for (int i = 0; i < ns; ++i) {
+        const StepLaunch &L = first[i];
+        tiles[i] = L.ep.nstrips * L.ep.nbands;
+        cls[i] = solve_class(std::max(L.lp.nx, L.lp.ny));
+        int irr_lines = 0;
+        for (int s = 0; s < 4; ++s) irr_lines += L.irr.count[s];
+        const int W = solve_class_waves(cls[i]);
+        blocks[i] = L.lp.nlines_x + L.lp.nlines_y + (irr_lines + W - 1) / W;
+        cells[i] = L.bp.ncells;
+        c.sig.insert(c.sig.end(), {ids[i], tiles[i], cls[i], blocks[i], cells[i]});
+    }
+    c.solve_order.resize(ns);
+    for (int i = 0; i < ns; ++i) c.solve_order[i] = i;
+    std::stable_sort(c.solve_order.begin(), c.solve_order.end(), [&](int a, int b) { return cls[a] < cls[b]; });
+    // starts buffer: explicit (ns+1), per class (n_c+1), boundary (ns+1)
+    std::vector<int> st;
+    c.exp_starts_off = 0;
+    st.push_back(0);
+    for (int i = 0; i < ns; ++i) st.push_back(st.back() + tiles[i]);
+    c.total_tiles = st.back();
+    int pos = 0;
+    for (int k = 0; k < 8; ++k) {
+        c.cls_first[k] = pos;
+        c.cls_starts_off[k] = (int)st.size();
+        st.push_back(0);
+        while (pos < ns && cls[c.solve_order[pos]] == k) {
+            st.push_back(st.back() + blocks[c.solve_order[pos]]);
+            ++pos;
+        }
+        c.cls_count[k] = pos - c.cls_first[k];
+        c.cls_blocks[k] = st.back();
+    }
+    c.bnd_starts_off = (int)st.size();
+    st.push_back(0);
+    for (int i = 0; i < ns; ++i) st.push_back(st.back() + cells[i]);
+    c.total_cells = st.back();
+    sync();
+    clearGraphs();
+    if (st.size() > starts_cap_) {
+        if (d_starts_) (void)hipFree(d_starts_);
+        void *mem = nullptr;
+        check(hipMalloc(&mem, st.size() * sizeof(int) * 2), "hipMalloc(batch lists)");
+        d_starts_ = static_cast<int *>(mem);
+        starts_cap_ = st.size() * 2;
+    }
+    check(hipMemcpy(d_starts_, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice), "upload batch lists");
+    sec_exp_ = 0;
+    sec_solve_ = align256((size_t)ns * sizeof(ExplicitJob));
+    sec_bnd_ = sec_solve_ + align256((size_t)ns * sizeof(SolveJob));
+    stride_ = sec_bnd_ + align256((size_t)ns * sizeof(BoundaryJob));
+    comp_ = std::move(c);
+    comp_valid_ = true;
+}
+
+// Launches of m steps whose parameter tables sit at blob + k * stride.
+void Batch::issue(const char *blob, size_t stride, int m)
+{
+    const int ns = (int)comp_.ids.size();
+    for (int k = 0; k < m; ++k) {
+        const char *b = blob + (size_t)k * stride;
+        const ExplicitJob *ej = reinterpret_cast<const ExplicitJob *>(b + sec_exp_);
+        const SolveJob *sj = reinterpret_cast<const SolveJob *>(b + sec_solve_);
+        const BoundaryJob *bj = reinterpret_cast<const BoundaryJob *>(b + sec_bnd_);
+        check(launch_explicit_batched(ej, d_starts_ + comp_.exp_starts_off, ns, comp_.total_tiles, stream_),
+              "explicit_batched");
+        for (int c = 0; c < 8; ++c)
+            if (comp_.cls_count[c] > 0)
+                check(launch_solve_batched(c, sj + comp_.cls_first[c], d_starts_ + comp_.cls_starts_off[c],
+                                           comp_.cls_count[c], comp_.cls_blocks[c], stream_), "solve_batched");
+        for (int pass = 1; pass >= 0; --pass)
+            check(launch_boundary_batched(bj, d_starts_ + comp_.bnd_starts_off, ns, comp_.total_cells, pass, stream_),
+                  "boundary_batched");
+    }
+}
+
+void Batch::run(int m, bool use_graph)
+{
+    const int ns = (int)scen_.size();
+    if (ns == 0 || m == 0) return;
+    std::vector<int> ids;
+    std::vector<ShallowWaterEngine *> eng;
+    for (auto &kv : scen_) { ids.push_back(kv.first); eng.push_back(kv.second.get()); }
+    // plan m steps of every scenario
+    std::vector<StepLaunch> plans((size_t)m * ns);
+    for (int k = 0; k < m; ++k)
+        for (int i = 0; i < ns; ++i) eng[i]->planStep(plans[(size_t)k * ns + i]);
+    for (const auto &L : plans)
+        if (L.scheme == CEL_CORRECTOR || L.ncopy > 0) throw Error(CEL_ESTATE, "corrector step in a batch");
+    {
+        std::vector<StepLaunch> first(plans.begin(), plans.begin() + ns);
+        std::vector<int> sig;
+        if (comp_valid_) {
+            // recompute the signature cheaply and compare
+            for (int i = 0; i < ns; ++i) {
+                const StepLaunch &L = first[i];
+                const int cls = solve_class(std::max(L.lp.nx, L.lp.ny));
+                int irr_lines = 0;
+                for (int s = 0; s < 4; ++s) irr_lines += L.irr.count[s];
+                const int W = solve_class_waves(cls);
+                sig.insert(sig.end(), {ids[i], L.ep.nstrips * L.ep.nbands, cls,
+                                       L.lp.nlines_x + L.lp.nlines_y + (irr_lines + W - 1) / W, L.bp.ncells});
+            }
+        }
+        if (!comp_valid_ || sig != comp_.sig) buildComposition(first, ids);
+    }
+    ensureBlob(stride_ * (size_t)std::max(m, std::max(graph_steps_, 1)));
+    // fill the host table: per step, explicit jobs in scenario order, solve jobs in class
+    // order, boundary jobs in scenario order
+    char *h = h_blob_[blob_slot_];
+    if (blob_used_[blob_slot_]) check(hipEventSynchronize(blob_ev_[blob_slot_]), "table event");
+    for (int k = 0; k < m; ++k) {
+        char *b = h + (size_t)k * stride_;
+        ExplicitJob *ej = reinterpret_cast<ExplicitJob *>(b + sec_exp_);
+        SolveJob *sj = reinterpret_cast<SolveJob *>(b + sec_solve_);
+        BoundaryJob *bj = reinterpret_cast<BoundaryJob *>(b + sec_bnd_);
+        for (int i = 0; i < ns; ++i) {
+            const StepLaunch &L = plans[(size_t)k * ns + i];
+            ej[i].p = L.ep;
+            ej[i].scheme = L.scheme;
+            ej[i].comp = eng[i]->compensated();
+            bj[i].p = L.bp;
+            bj[i].p.clock = nullptr;
+        }
+        for (int q = 0; q < ns; ++q) {
+            const int i = comp_.solve_order[q];
+            const StepLaunch &L = plans[(size_t)k * ns + i];
+            sj[q].p = L.lp;
+            sj[q].irr = L.irr;
+            sj[q].irr.clock = nullptr;
+            sj[q].comp = eng[i]->compensated();
+        }
+    }
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        for (const auto &g : graphs_)
+            if (g.m == m) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                issue(d_blob_, stride_, m);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream_, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            check(e, "hipGraphInstantiate");
+            graphs_.push_back(GraphEntry{m, exec});
+        }
+    }
+    check(hipMemcpyAsync(d_blob_, h, stride_ * (size_t)m, hipMemcpyHostToDevice, stream_), "upload batch table");
+    check(hipEventRecord(blob_ev_[blob_slot_], stream_), "table event");
+    blob_used_[blob_slot_] = true;
+    blob_slot_ ^= 1;
+    if (exec) check(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
+    else issue(d_blob_, stride_, m);
+}
+
+void Batch::step(int nsteps)
+{
+    if (nsteps < 0) throw Error(CEL_EINVAL, "negative step count");
+    int done = 0;
+    if (graph_steps_ > 0)
+        for (; done + graph_steps_ <= nsteps; done += graph_steps_) run(graph_steps_, true);
+    // remainder: one table, direct launches (chunks bounded by the table's step capacity)
+    while (done < nsteps) {
+        const int m = std::min(nsteps - done, std::max(graph_steps_, 16));
+        run(m, false);
+        done += m;
+    }
+}
+
+float Batch::timedSteps(int nsteps)
+{
+    check(hipEventRecord(ev_[0], stream_), "event");
+    step(nsteps);
+    check(hipEventRecord(ev_[1], stream_), "event");
+    check(hipEventSynchronize(ev_[1]), "event sync");
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event");
+    return ms;
+}
+
+}  // namespace cel

@@ -1,0 +1,86 @@
+// batch.hpp -- many independent scenarios advanced together on one GPU.
+//
+// Each scenario is a ShallowWaterEngine (its own device state, tables and step state
+// machine).  A batch step plans every scenario's step on the host, writes the parameter
+// blocks of all scenarios into one device table, and issues one launch per kernel over
+// a work-list concatenated across scenarios: explicit tiles, line-solve blocks (one
+// launch per line-length class), boundary cells.  Blocks of K steps are captured as one
+// hipGraph per batch composition and replayed with that block's parameter table, so
+// scenarios at different points of their schedules (bootstrap, steady state) share a
+// graph.  Results equal those of each scenario run alone.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace cel {
+
+hipError_t launch_explicit_batched(const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
+                                   hipStream_t st);
+hipError_t launch_solve_batched(int cls, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
+                                hipStream_t st);
+hipError_t launch_boundary_batched(const BoundaryJob *jobs, const int *starts, int njobs, int total_cells,
+                                   int mirror_pass, hipStream_t st);
+int solve_class(int nmax);
+int solve_class_waves(int cls);
+
+class Batch {
+public:
+    Batch();
+    ~Batch();
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    int add(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3, const float *state3);
+    void remove(int id);
+    void step(int nsteps);
+    void setGraphSteps(int k);
+    void sync();
+    float timedSteps(int nsteps);
+    ShallowWaterEngine &scenario(int id);      // synchronises the batch stream first
+    int count() const { return (int)scen_.size(); }
+    size_t deviceBytes() const;
+    void ids(int *out) const;
+
+private:
+    struct Composition {
+        std::vector<int> ids;                  // scenario order of the explicit/boundary lists
+        std::vector<int> solve_order;          // scenario positions sorted by line-length class
+        std::vector<int> sig;                  // per-scenario work counts and classes
+        int total_tiles = 0, total_cells = 0;
+        int cls_first[8] = {}, cls_count[8] = {}, cls_blocks[8] = {};
+        int cls_starts_off[8] = {};            // offsets into the starts buffer (ints)
+        int exp_starts_off = 0, bnd_starts_off = 0;
+    };
+    void run(int m, bool use_graph);
+    void buildComposition(const std::vector<StepLaunch> &first, const std::vector<int> &ids);
+    void issue(const char *blob, size_t stride, int m);
+    void clearGraphs();
+    void check(hipError_t e, const char *what) const;
+    void ensureBlob(size_t bytes);
+
+    hipStream_t stream_ = nullptr;
+    std::map<int, std::unique_ptr<ShallowWaterEngine>> scen_;
+    int next_id_ = 1;
+    int graph_steps_ = 0;
+    Composition comp_;
+    bool comp_valid_ = false;
+    int *d_starts_ = nullptr;
+    size_t starts_cap_ = 0;
+    // per-step parameter table: device copy and two pinned host buffers
+    char *d_blob_ = nullptr;
+    char *h_blob_[2] = {};
+    size_t blob_cap_ = 0;
+    hipEvent_t blob_ev_[2] = {};
+    bool blob_used_[2] = {};
+    int blob_slot_ = 0;
+    size_t sec_exp_ = 0, sec_solve_ = 0, sec_bnd_ = 0, stride_ = 0;   // blob layout of one step
+    struct GraphEntry { int m; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs_;
+    hipEvent_t ev_[2] = {};
+};
+
+}  // namespace cel

@@ -4,6 +4,7 @@
 #include <new>
 #include <string>
 
+#include "batch.hpp"
 #include "engine.hpp"
 
 namespace cel {
@@ -13,6 +14,10 @@ void initial_state(const cel_settings &s, const float *bottom3, const cel_solito
 
 struct cel_engine {
     cel::ShallowWaterEngine *impl;
+};
+
+struct cel_batch {
+    cel::Batch *impl;
 };
 
 namespace {
@@ -44,6 +49,12 @@ cel::ShallowWaterEngine &eng(cel_engine *e)
 {
     if (!e || !e->impl) throw cel::Error(CEL_EINVAL, "null engine");
     return *e->impl;
+}
+
+cel::Batch &bat(cel_batch *b)
+{
+    if (!b || !b->impl) throw cel::Error(CEL_EINVAL, "null batch");
+    return *b->impl;
 }
 }  // namespace
 
@@ -158,6 +169,100 @@ int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3)
     return guarded([&] {
         if (!abc_x3 || !abc_y3) throw cel::Error(CEL_EINVAL, "null output");
         eng(e).coefficients(abc_x3, abc_y3);
+    });
+}
+
+int cel_estimate_device_bytes(const cel_settings *s, const cel_boundary bnd[4], size_t *bytes)
+{
+    return guarded([&] {
+        if (!s || !bnd || !bytes) throw cel::Error(CEL_EINVAL, "null argument");
+        *bytes = cel::ShallowWaterEngine::estimateBytes(*s, bnd);
+    });
+}
+
+int cel_batch_create(cel_batch **out)
+{
+    return guarded([&] {
+        if (!out) throw cel::Error(CEL_EINVAL, "null output");
+        *out = nullptr;
+        auto *b = new cel_batch{nullptr};
+        try {
+            b->impl = new cel::Batch();
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int cel_batch_destroy(cel_batch *b)
+{
+    return guarded([&] {
+        if (!b) return;
+        delete b->impl;
+        delete b;
+    });
+}
+
+int cel_batch_add(cel_batch *b, const cel_settings *s, const cel_boundary bnd[4], const float *bottom3,
+                  const float *state3, int *id)
+{
+    return guarded([&] {
+        if (!s || !id) throw cel::Error(CEL_EINVAL, "null settings or output");
+        *id = bat(b).add(*s, bnd, bottom3, state3);
+    });
+}
+
+int cel_batch_remove(cel_batch *b, int id) { return guarded([&] { bat(b).remove(id); }); }
+int cel_batch_step(cel_batch *b, int nsteps) { return guarded([&] { bat(b).step(nsteps); }); }
+int cel_batch_set_graph_steps(cel_batch *b, int steps) { return guarded([&] { bat(b).setGraphSteps(steps); }); }
+int cel_batch_sync(cel_batch *b) { return guarded([&] { bat(b).sync(); }); }
+
+int cel_batch_timed_steps(cel_batch *b, int nsteps, float *ms)
+{
+    return guarded([&] {
+        if (!ms) throw cel::Error(CEL_EINVAL, "null output");
+        *ms = bat(b).timedSteps(nsteps);
+    });
+}
+
+int cel_batch_count(cel_batch *b, int *n)
+{
+    return guarded([&] {
+        if (!n) throw cel::Error(CEL_EINVAL, "null output");
+        *n = bat(b).count();
+    });
+}
+
+int cel_batch_device_bytes(cel_batch *b, size_t *bytes)
+{
+    return guarded([&] {
+        if (!bytes) throw cel::Error(CEL_EINVAL, "null output");
+        *bytes = bat(b).deviceBytes();
+    });
+}
+
+int cel_batch_get_state(cel_batch *b, int id, float *state3)
+{
+    return guarded([&] { bat(b).scenario(id).getState(state3); });
+}
+
+int cel_batch_get_state_f64(cel_batch *b, int id, double *state3)
+{
+    return guarded([&] { bat(b).scenario(id).getStateF64(state3); });
+}
+
+int cel_batch_stats(cel_batch *b, int id, cel_stats *out)
+{
+    return guarded([&] { bat(b).scenario(id).stats(out); });
+}
+
+int cel_batch_time(cel_batch *b, int id, double *t)
+{
+    return guarded([&] {
+        if (!t) throw cel::Error(CEL_EINVAL, "null output");
+        *t = bat(b).scenario(id).time();
     });
 }
 

@@ -140,6 +140,31 @@ void ShallowWaterEngine::allocate()
     h_stats_ = static_cast<double *>(mem);
 }
 
+size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_boundary bnd[4])
+{
+    const int X = s.nx + 4, Y = s.ny + 4;
+    const size_t P = round_up((size_t)X + 16, 64), PT = round_up((size_t)Y + 16, 64);
+    const bool comp = s.state_precision == CEL_STATE_COMPENSATED;
+    const size_t plane = P * Y * sizeof(float), tplane = PT * X * sizeof(float);
+    size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 2 + (comp ? 2 : 0));
+    const int tiles = ((4 * (s.nx / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (s.ny / 4) + kStatsTileY - 1) / kStatsTileY);
+    b += ((size_t)tiles + 1) * kStatsValues * sizeof(double);
+    for (int side = 0; side < 4; ++side) {
+        const cel_boundary &c = bnd[side];
+        const size_t ncell = (size_t)c.width * ((side == CEL_WEST || side == CEL_EAST) ? Y : X);
+        if (ncell == 0) continue;
+        if (c.type == CEL_SPONGE) b += ncell * 4;
+        else if (c.type == CEL_SINEWAVE) b += ncell * 5 * 4;
+        else if (c.type == CEL_IRREGULARWAVES) {
+            const int L = count_of_matrices((side == CEL_WEST || side == CEL_EAST) ? s.nx : s.ny);
+            const int nact = std::min(c.n_waves, 1 << (L - 1));
+            const size_t nw1 = (size_t)std::max(nact, 1);
+            b += ncell * nw1 * 16 + nw1 * 12 + ncell + ncell * 12;
+        }
+    }
+    return b;
+}
+
 void ShallowWaterEngine::uploadBottom(const float *bottom3)
 {
     const size_t N = (size_t)X_ * Y_;

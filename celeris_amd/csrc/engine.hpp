@@ -84,6 +84,9 @@ public:
     void setProfiling(bool on);
     bool kernelTime(const std::string &name, float *ms, int *launches);
     size_t deviceBytes() const { return arena_bytes_ + table_bytes_; }
+    bool compensated() const { return comp_; }
+    // device bytes an engine with these settings holds after creation (admission control)
+    static size_t estimateBytes(const cel_settings &s, const cel_boundary bnd[4]);
     void coefficients(float *ax, float *ay);
     void stats(cel_stats *out);
     void resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out);

@@ -538,7 +538,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 
 // Irregular waves: one wave per (side, strip index, line cell) computes that cell's
 // pairwise-tree sum of the component texels (ColumnSumReduce/RowSumReduce order).
-__device__ void irregular_line(const IrregularSide &S, int strip, int line, float t)
+__device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip, int line, float t)
 {
     const int lane = threadIdx.x & 63;
     const int cell = strip * S.nlines + line;
@@ -646,11 +646,12 @@ __device__ __forceinline__ void load_chunk(const float *src, int base, int n, fl
 // Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
 // D, K, Q point at element 0 of the line (16-byte aligned); out at element 0 of the
 // ghost-inclusive output line + 2 (8-byte aligned).
-// COMP: the solution is an increment; out/out_lo receive the two-float sum of the previous
+// comp: the solution is an increment; out/out_lo receive the two-float sum of the previous
 // state (prev, prev_lo, same layout as out) and the increment.
-template <int C, int W, bool COMP>
-__device__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out, LineSmem<C, W> &sm,
-                              const float *prev, const float *prev_lo, float *out_lo)
+template <int C, int W>
+__device__ __forceinline__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out,
+                                              LineSmem<C, W> &sm, bool comp, const float *prev, const float *prev_lo,
+                                              float *out_lo)
 {
     constexpr int NT = 64 * W;
     constexpr int K0 = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : C == 16 ? 4 : 5;
@@ -735,7 +736,7 @@ __device__ void cr_line_block(int n, const float *D, const float *K, const float
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
-    if (COMP) {
+    if (comp) {
 #pragma unroll
         for (int i = 0; i < C; ++i)
             if (base + i < n) {
@@ -779,17 +780,13 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     }
     const bool xdir = bid < p.nlines_x;
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
-    if (xdir) {
-        const size_t o = (size_t)line * p.P;
-        cr_line_block<C, W, COMP>(p.nx, p.Dx + o, p.Kx + o, p.Qx + o, p.hu_out + o + 2, sm,
-                                  COMP ? p.hu_prev + o + 2 : nullptr, COMP ? p.hu_prev_lo + o + 2 : nullptr,
-                                  COMP ? p.hu_lo_out + o + 2 : nullptr);
-    } else {
-        const size_t o = (size_t)line * p.PT;
-        cr_line_block<C, W, COMP>(p.ny, p.DyT + o, p.KyT + o, p.QyT + o, p.hvT_out + o + 2, sm,
-                                  COMP ? p.hvT_prev + o + 2 : nullptr, COMP ? p.hvT_prev_lo + o + 2 : nullptr,
-                                  COMP ? p.hvT_lo_out + o + 2 : nullptr);
-    }
+    const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
+    const int n = xdir ? p.nx : p.ny;
+    const float *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    float *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
+    cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, COMP, COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
+                        COMP ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
+                        COMP ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
 }
 
 // ------------------------------------------------------------------ boundaries
@@ -830,7 +827,7 @@ __device__ __forceinline__ Val pre_value(const BoundaryParams &p, int x, int y)
 }
 
 // Boundary cell tid of one scenario (tid < p.ncells).
-__device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid)
+__device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, int mirror_pass)
 {
     // enumerate the boundary cell set: left band, right band, then bottom/top bands between them
     int x, y;
@@ -872,7 +869,7 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid)
             }
         }
     }
-    if ((int)mirrored != p.mirror_pass) return;
+    if ((int)mirrored != mirror_pass) return;
     Val v{0.f, 0.f, 0.f};
     if (have_source && cx >= 0 && cx < p.X && cy >= 0 && cy < p.Y) v = pre_value(p, cx, cy);
 #pragma unroll
@@ -933,7 +930,92 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid)
 __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid < p.ncells) boundary_cell(p, tid);
+    if (tid < p.ncells) boundary_cell(p, tid, p.mirror_pass);
+}
+
+// ------------------------------------------------------------------ batched scenarios
+//
+// Many independent scenarios advance in one launch per kernel: the launch's blocks (or
+// threads) are a work-list concatenated over the scenarios, and each finds its scenario
+// by binary search over the list's start offsets.  Parameter blocks live in device
+// memory, one per scenario and step.
+
+// largest j with starts[j] <= v (starts[0] == 0, nondecreasing)
+__device__ __forceinline__ int find_job(const int *starts, int njobs, int v)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void explicit_batched(const ExplicitJob *__restrict__ jobs,
+                                                      const int *__restrict__ starts, int njobs)
+{
+    __shared__ ExplicitSmem<false> sm;
+    const int total = starts[njobs];
+    const int per_xcd = (total + 7) / 8;
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= total) return;
+    const int j = find_job(starts, njobs, t);
+    const ExplicitJob &J = jobs[j];
+    const int tile = t - starts[j];
+    switch (J.scheme * 2 + J.comp) {
+    case 0: explicit_tile<0, false>(J.p, tile, sm); break;
+    case 1: explicit_tile<0, true>(J.p, tile, sm); break;
+    case 2: explicit_tile<1, false>(J.p, tile, sm); break;
+    case 3: explicit_tile<1, true>(J.p, tile, sm); break;
+    case 6: explicit_tile<3, false>(J.p, tile, sm); break;
+    case 7: explicit_tile<3, true>(J.p, tile, sm); break;
+    default: break;   // corrector steps are not batched (rejected on the host)
+    }
+}
+
+template <int C, int W>
+__global__ __launch_bounds__(64 * W) void solve_batched(const SolveJob *__restrict__ jobs, const int *__restrict__ starts,
+                                                       int njobs)
+{
+    __shared__ LineSmem<C, W> sm;
+    const int j = find_job(starts, njobs, blockIdx.x);
+    const SolveJob &J = jobs[j];
+    const LineParams &p = J.p;
+    const IrregularAll &irr = J.irr;
+    const int bid = blockIdx.x - starts[j];
+    const int nlines = p.nlines_x + p.nlines_y;
+    if (bid >= nlines) {
+        int idx = (bid - nlines) * W + (threadIdx.x >> 6);
+        for (int s = 0; s < 4; ++s) {
+            if (idx < irr.count[s]) {
+                const IrregularSide &S = irr.side[s];
+                irregular_line(S, idx / S.nlines, idx % S.nlines, irr.t);
+                return;
+            }
+            idx -= irr.count[s];
+        }
+        return;
+    }
+    const bool xdir = bid < p.nlines_x;
+    const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
+    const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
+    const int n = xdir ? p.nx : p.ny;
+    const float *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    float *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
+    const bool comp = J.comp != 0;
+    cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
+                        comp ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
+                        comp ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
+}
+
+__global__ __launch_bounds__(256) void boundary_batched(const BoundaryJob *__restrict__ jobs,
+                                                        const int *__restrict__ starts, int njobs,
+                                                        int mirror_pass)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= starts[njobs]) return;
+    const int j = find_job(starts, njobs, tid);
+    boundary_cell(jobs[j].p, tid - starts[j], mirror_pass);
 }
 
 // ------------------------------------------------------------------ diagnostics
@@ -1151,6 +1233,52 @@ hipError_t launch_solve(bool comp, const LineParams &p, const IrregularAll &irr,
         return hipGetLastError();
     }
     return hipErrorInvalidValue;   // lines longer than CEL_MAX_LINE are rejected at creation
+}
+
+hipError_t launch_explicit_batched(const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
+                                   hipStream_t st)
+{
+    if (total_tiles > 0)
+        hipLaunchKernelGGL(explicit_batched, dim3(8 * ((total_tiles + 7) / 8)), dim3(64), 0, st, jobs, starts, njobs);
+    return hipGetLastError();
+}
+
+int solve_class(int nmax)
+{
+    return nmax <= 64 ? 0 : nmax <= 128 ? 1 : nmax <= 256 ? 2 : nmax <= 512 ? 3 : nmax <= 1024 ? 4 : nmax <= 2048 ? 5
+         : nmax <= 4096 ? 6 : 7;
+}
+
+// threads of one line per class: 64 W
+int solve_class_waves(int cls) { static const int w[8] = {1, 1, 1, 1, 2, 4, 4, 8}; return w[cls]; }
+
+hipError_t launch_solve_batched(int cls, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
+                                hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    const dim3 grid(total_blocks);
+#define CEL_SB(C, W) hipLaunchKernelGGL((solve_batched<C, W>), grid, dim3(64 * (W)), 0, st, jobs, starts, njobs)
+    switch (cls) {
+    case 0: CEL_SB(1, 1); break;
+    case 1: CEL_SB(2, 1); break;
+    case 2: CEL_SB(4, 1); break;
+    case 3: CEL_SB(8, 1); break;
+    case 4: CEL_SB(8, 2); break;
+    case 5: CEL_SB(8, 4); break;
+    case 6: CEL_SB(16, 4); break;
+    default: CEL_SB(16, 8); break;
+    }
+#undef CEL_SB
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_batched(const BoundaryJob *jobs, const int *starts, int njobs, int total_cells,
+                                   int mirror_pass, hipStream_t st)
+{
+    if (total_cells > 0)
+        hipLaunchKernelGGL(boundary_batched, dim3((total_cells + 255) / 256), dim3(256), 0, st, jobs, starts, njobs,
+                           mirror_pass);
+    return hipGetLastError();
 }
 
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)

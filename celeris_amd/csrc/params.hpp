@@ -125,6 +125,20 @@ struct BoundaryParams {
     SideParams side[4];
 };
 
+// One scenario's part of a batched launch (parameters of one step).
+struct ExplicitJob {
+    ExplicitParams p;
+    int scheme, comp;
+};
+struct SolveJob {
+    LineParams p;
+    IrregularAll irr;
+    int comp;
+};
+struct BoundaryJob {
+    BoundaryParams p;
+};
+
 // GetStats (compute.hlsl:1319-1378): cells [2, 2 + ncx) x [2, 2 + ncy), tiles of
 // kStatsTileX x kStatsTileY cells per block, nine partials per block.
 constexpr int kStatsTileX = 64, kStatsTileY = 16, kStatsValues = 9;

@@ -208,6 +208,38 @@ int cel_engine_stats(cel_engine *e, cel_stats *out);
  * settings' timestep, or the last cel_engine_set_dt).  out may be NULL. */
 int cel_engine_reset_timestep(cel_engine *e, const cel_adaptive *a, cel_stats *out);
 
+/* Device bytes an engine with these settings and boundaries holds once created
+ * (admission control against an HBM budget). */
+int cel_estimate_device_bytes(const cel_settings *s, const cel_boundary bnd[4], size_t *bytes);
+
+/* ---- Batched scenarios.  No counterpart in the reference (one engine per process).
+ * A batch owns independent scenarios of any grid size and advances them together: one
+ * launch per kernel per step over a work-list spanning every scenario (explicit tiles,
+ * line-solve blocks per line-length class, boundary cells), optionally replayed from
+ * hipGraphs of several steps.  Each scenario's results are those of a cel_engine with
+ * the same inputs.  Scenarios cannot take corrector steps. */
+typedef struct cel_batch cel_batch;
+int cel_batch_create(cel_batch **out);
+int cel_batch_destroy(cel_batch *b);
+/* Adds a scenario (as cel_engine_create) and returns its id (> 0). */
+int cel_batch_add(cel_batch *b, const cel_settings *s, const cel_boundary bnd[4], const float *bottom3,
+                  const float *state3, int *id);
+int cel_batch_remove(cel_batch *b, int id);
+/* Advances every scenario by nsteps (asynchronous). */
+int cel_batch_step(cel_batch *b, int nsteps);
+/* Steps per captured block (0: direct launches). */
+int cel_batch_set_graph_steps(cel_batch *b, int steps);
+int cel_batch_sync(cel_batch *b);
+/* Runs nsteps and returns the elapsed device time (HIP events on the batch stream). */
+int cel_batch_timed_steps(cel_batch *b, int nsteps, float *ms);
+int cel_batch_count(cel_batch *b, int *n);
+int cel_batch_device_bytes(cel_batch *b, size_t *bytes);
+/* Per-scenario access (synchronises the batch first). */
+int cel_batch_get_state(cel_batch *b, int id, float *state3);
+int cel_batch_get_state_f64(cel_batch *b, int id, double *state3);
+int cel_batch_stats(cel_batch *b, int id, cel_stats *out);
+int cel_batch_time(cel_batch *b, int id, double *t);
+
 /* Level-0 tridiagonal coefficients as the engine uses them (engine.cpp:2638-2694,
  * 2781-2826): three planes A, B, C for x and for y, computed on the device. */
 int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3);

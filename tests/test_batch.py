@@ -1,0 +1,103 @@
+"""Batched scenarios (cel_batch_*) against the same scenarios run alone.
+
+A batch issues one launch per kernel per step over a work-list spanning all scenarios, with
+parameter blocks from a device table; each scenario must end bit-identical to a
+cel_engine given the same inputs: uneven grid sizes (every line-length class up to 1024),
+every boundary type, compensated and plain state, scenarios joining and leaving
+mid-run (so members sit at different points of their schedules), captured blocks and
+direct launches.
+"""
+import numpy as np
+import pytest
+
+from celeris_amd import (IRREGULARWAVES, SINEWAVE, SOLID, SPONGE, STATE_COMPENSATED, UNIFORMTIMESERIES, Batch,
+                         Boundary, CeleriError, ShallowWaterEngine, estimate_device_bytes)
+from celeris_amd import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(seed=0):
+    rng = np.random.default_rng(seed)
+    waves = np.stack([rng.uniform(0.005, 0.02, 5), rng.uniform(1.0, 3.0, 5), rng.uniform(-0.4, 0.4, 5),
+                      rng.uniform(0, 2 * np.pi, 5)], axis=1).astype(np.float32)
+    t = np.arange(0, 40, 0.05, dtype=np.float32)
+    uts = np.stack([t, 0.02 * np.sin(t), 0.01 * np.cos(t), 0 * t], axis=1).astype(np.float32)
+    out = [
+        S.small(nx=40, ny=36, seed=1),
+        S.small(nx=130, ny=70, seed=2, bnd=[Boundary(SINEWAVE, 2, amplitude=0.03, period=1.5), Boundary(SPONGE, 10),
+                                            Boundary(SOLID, 2), Boundary(SOLID, 2)]),
+        S.small(nx=64, ny=64, seed=3, bnd=[Boundary(IRREGULARWAVES, 2, waves=waves), Boundary(SPONGE, 8),
+                                           Boundary(UNIFORMTIMESERIES, 2, uts=uts), Boundary(SOLID, 2)]),
+        S.small(nx=260, ny=150, seed=4),
+        S.small(nx=300, ny=40, seed=5, time_scheme=3),
+        S.small(nx=600, ny=24, seed=6),
+        S.small(nx=20, ny=1000, seed=7),
+    ]
+    out[3].s.state_precision = STATE_COMPENSATED
+    return out
+
+
+def alone(sc, steps):
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(steps)
+    st = e.get_state_f64()
+    t = e.time
+    e.close()
+    return st, t
+
+
+@pytest.mark.parametrize("graph_steps", [0, 25])
+def test_batch_equals_scenarios_alone(graph_steps):
+    scs = mix()
+    b = Batch(graph_steps=graph_steps)
+    ids = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs]
+    assert len(b) == len(scs)
+    b.step(137)
+    for sid, sc in zip(ids, scs):
+        ref, t = alone(sc, 137)
+        got = b.get_state_f64(sid)
+        assert np.isfinite(got).all()
+        assert b.time(sid) == t
+        assert np.array_equal(got, ref), f"scenario {sc.name} differs"
+    b.close()
+
+
+def test_batch_join_and_leave_midrun():
+    scs = mix(1)
+    b = Batch(graph_steps=10)
+    first = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs[:4]]
+    b.step(35)                                   # 3 blocks + 5 direct steps
+    b.remove(first[1])
+    late = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs[4:]]
+    b.step(47)
+    for sid, sc, n in [(first[0], scs[0], 82), (first[2], scs[2], 82), (first[3], scs[3], 82),
+                       (late[0], scs[4], 47), (late[1], scs[5], 47), (late[2], scs[6], 47)]:
+        ref, _ = alone(sc, n)
+        assert np.array_equal(b.get_state_f64(sid), ref), sc.name
+    with pytest.raises(CeleriError):
+        b.get_state(first[1])
+    b.close()
+
+
+def test_batch_stats_and_bytes():
+    scs = mix(2)[:3]
+    b = Batch()
+    ids = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs]
+    b.step(20)
+    for sid, sc in zip(ids, scs):
+        e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        assert e.device_bytes() == estimate_device_bytes(sc.s, sc.boundaries)
+        e.timestep(20)
+        assert b.stats(sid) == e.stats()
+        e.close()
+    assert b.device_bytes() >= sum(estimate_device_bytes(sc.s, sc.boundaries) for sc in scs)
+    b.close()
+
+
+def test_batch_rejects_corrector():
+    sc = S.small(nx=24, ny=24, correction_steps=1)
+    b = Batch()
+    with pytest.raises(CeleriError):
+        b.add(sc.s, sc.boundaries, sc.bottom, sc.state)
+    b.close()
