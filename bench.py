@@ -8,8 +8,15 @@ stencil, x- and y-line tridiagonal solves, boundaries) over the whole grid; inpu
 resident in HBM before the timed region.
 
 Prints one JSON line: Gcell-steps/s (interior cells x steps / wall time), the dominant
-kernel's achieved HBM bandwidth against MI355X's 8 TB/s, and the CPU oracle timed on a
-bounded sample of the same workload.
+kernel's achieved HBM bandwidth against MI355X's 8 TB/s, the CPU oracle timed on a
+bounded sample of the same workload, and a bounded sample of the serving stream (Poisson
+arrivals of scenarios with sides 256-2048 and 500-20000 steps, continuous batching) with
+p50/p99 completion latency beside the plain engine serving the same stream one request at
+a time.
+
+--workload ensemble prints the batched-ensemble line instead (64 scenarios with sides
+log-uniform in [128, 1024], compensated update, 50-step graph blocks) against the same
+scenarios run one at a time with the plain engine.
 """
 import argparse
 import json
@@ -46,6 +53,65 @@ def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
                       f"{dt:.1f} s, OpenMP {threads} threads"}
 
 
+def serving_leg(n, rate, seed, block):
+    """A bounded sample of the serving stream: n requests (generated before the clock
+    starts), served in real time with continuous batching, then replayed one at a time."""
+    from celeris_amd import scenarios as S
+    from celeris_amd import serving
+    reqs = S.serving_requests(n, rate, seed=seed)
+    _, batched, _ = serving.serve(reqs, block=block)
+    _, alone = serving.one_at_a_time(reqs)
+    sides = {}
+    for r in reqs:
+        sides[r.scenario.s.nx] = sides.get(r.scenario.s.nx, 0) + 1
+    return {"workload": f"{n} requests, Poisson arrivals at {rate}/s, square grids {dict(sorted(sides.items()))}, "
+                        "500-20000 steps (log-uniform), sloped seabed + GRF, sine/irregular forcing, "
+                        f"compensated update, {block}-step blocks",
+            "batched": batched, "one_at_a_time": alone,
+            "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
+
+
+def ensemble_line(args):
+    from celeris_amd import Batch, ShallowWaterEngine
+    from celeris_amd import scenarios as S
+    members = S.ensemble(args.members, seed=4, steps=args.steps)
+    cells = sum(m.s.nx * m.s.ny for m in members)
+    b = Batch(graph_steps=args.graph_steps)
+    ids = [b.add(m.s, m.boundaries, m.bottom, m.state) for m in members]
+    b.step(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    ms = b.timed_steps(args.steps)
+    wall = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(b.get_state(i)).all()) for i in ids)
+    b.close()
+    batched = cells * args.steps / wall / 1e9
+    # the same members one at a time with the plain engine (step-by-step launches)
+    alone_wall = 0.0
+    for m in members:
+        e = ShallowWaterEngine(m.s, m.boundaries, m.bottom, m.state)
+        e.timestep(args.warmup)
+        e.sync()
+        t0 = time.perf_counter()
+        e.timed_steps(args.steps)
+        alone_wall += time.perf_counter() - t0
+        e.close()
+    alone = cells * args.steps / alone_wall / 1e9
+    sides = [(m.s.nx, m.s.ny) for m in members]
+    return {
+        "metric": "Gcell-steps/s (batched ensemble)", "value": batched, "unit": "Gcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(members)} scenarios, sides log-uniform in [128, 1024] "
+                               f"({sum(nx * ny for nx, ny in sides)} cells), dx 0.5 m, sloped seabed + GRF, "
+                               "sine/irregular forcing, compensated update",
+                   "graph_steps": args.graph_steps, "event_ms_per_step": ms / args.steps,
+                   "one_at_a_time_gcell_steps_per_s": alone, "speedup_vs_one_at_a_time": batched / alone,
+                   "state_finite": finite, "smallest": min(sides, key=lambda v: v[0] * v[1]),
+                   "largest": max(sides, key=lambda v: v[0] * v[1])},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +122,11 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=50,
                     help="steps per captured hipGraph block (0: launch step by step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["headline", "ensemble"], default="headline")
+    ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests in the serving sample of the headline line (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -73,6 +144,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding of the step is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ensemble":
+        print(json.dumps(ensemble_line(args)), flush=True)
+        return
 
     from celeris_amd import ShallowWaterEngine
     from celeris_amd import scenarios as S
@@ -128,6 +203,8 @@ def main():
                      "bytes_per_launch": BYTES_PER_CELL_STEP * cells},
         "cpu_baseline": None,
     }
+    if args.serving_requests > 0:
+        out["serving"] = serving_leg(args.serving_requests, args.serving_rate, 5, args.graph_steps or 50)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

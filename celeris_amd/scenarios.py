@@ -10,8 +10,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (EAST, IRREGULARWAVES, NORTH, SINEWAVE, SOLID, SOUTH, SPONGE, WEST, Boundary, bottom_from_corners,
-               initial_state, settings)
+from . import (EAST, IRREGULARWAVES, NORTH, SINEWAVE, SOLID, SOUTH, SPONGE, STATE_COMPENSATED, STATE_FP32, WEST,
+               Boundary, bottom_from_corners, initial_state, settings)
 
 G = 9.81
 
@@ -146,5 +146,71 @@ def small(nx=48, ny=40, dx=0.5, seed=0, bnd=None, soliton=True, steps=100, corre
     return _finish(f"small{nx}x{ny}", s, bnd, z, steps, solitons=sol)
 
 
-__all__ = ["Scenario", "plain_beach", "headline", "solitary_channel", "small", "jonswap_waves",
+def ensemble_member(rng, nx=None, ny=None, side_range=(128, 1024), dx=0.5, compensated=True, steps=5000,
+                    name="member"):
+    """One scenario of the batched ensemble: sides log-uniform in side_range (unless given), a
+    seabed rising from h0 (1.5-3 m) to +0.5 m dry land at the east edge with a slope of 1:15
+    to 1:60 (steepened where needed so that the beach spans at most 60 % of the domain and
+    the west boundary sits at depth h0), plus a Gaussian random field (std 0.1-0.5 m,
+    correlation 20 cells); west forcing is a sine wave or 32 JONSWAP components (amplitude
+    0.02-0.2 m, period 1-6 s; the amplitude is capped at 3 % of the deep-water wavelength and
+    5 % of h0 so that the non-breaking model stays in range); solid walls elsewhere;
+    compensated state update."""
+    lo, hi = np.log(side_range[0]), np.log(side_range[1])
+    nx = nx or int(round(np.exp(rng.uniform(lo, hi))))
+    ny = ny or int(round(np.exp(rng.uniform(lo, hi))))
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    X, _ = corner_coords(nx, ny, W, L)
+    h0 = rng.uniform(1.5, 3.0)
+    slope = min(rng.uniform(15.0, 60.0), 0.6 * W / (h0 + 0.5))
+    std = rng.uniform(0.1, 0.5)
+    z = np.maximum(-h0, 0.5 - (W - X) / slope) + gaussian_random_field(X.shape, 20.0, std, rng)
+    period = rng.uniform(1.0, 6.0)
+    amp = min(rng.uniform(0.02, 0.2), 0.03 * G * period ** 2 / (2 * np.pi), 0.05 * h0)
+    if rng.random() < 0.5:
+        west = Boundary(SINEWAVE, 2, 0.0, amplitude=amp, period=period, theta=rng.uniform(-0.3, 0.3))
+        forcing = "sine"
+    else:
+        west = Boundary(IRREGULARWAVES, 2, 0.0, waves=jonswap_waves(32, 2 * np.sqrt(2) * amp, period, 3.3, rng))
+        forcing = "irregular"
+    hmax = max(float(-z.min()), 0.5)
+    s = settings(nx, ny, W, L, cfl_dt(dx, hmax), theta=1.3,
+                 state_precision=STATE_COMPENSATED if compensated else STATE_FP32)
+    bnd = [west, Boundary(SOLID, 2), Boundary(SOLID, 2), Boundary(SOLID, 2)]
+    return _finish(name, s, bnd, z, steps, info={"slope": slope, "h0": h0, "grf_std": std, "amp": amp,
+                                                 "period": period, "forcing": forcing})
+
+
+def ensemble(n=64, seed=4, steps=5000, compensated=True):
+    """The batched-ensemble workload: n independent members of uneven size."""
+    rng = np.random.default_rng(seed)
+    return [ensemble_member(rng, steps=steps, compensated=compensated, name=f"member{i}") for i in range(n)]
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the stream starts
+    scenario: Scenario
+    steps: int
+
+
+def serving_requests(n, rate, seed=5, sides=(256, 512, 1024, 2048), weights=(0.4, 0.3, 0.2, 0.1),
+                     steps_range=(500, 20000), compensated=True):
+    """Scenario requests as a Poisson process of the given rate (1/s): square grids with the
+    side drawn from `sides` with `weights`, durations log-uniform in steps_range, seabed and
+    forcing drawn as for the ensemble."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        side = int(rng.choice(sides, p=np.asarray(weights) / np.sum(weights)))
+        steps = int(round(np.exp(rng.uniform(np.log(steps_range[0]), np.log(steps_range[1])))))
+        sc = ensemble_member(rng, nx=side, ny=side, steps=steps, compensated=compensated, name=f"req{i}")
+        out.append(Request(i, t, sc, steps))
+    return out
+
+
+__all__ = ["ensemble_member", "ensemble", "Request", "serving_requests", "Scenario", "plain_beach", "headline", "solitary_channel", "small", "jonswap_waves",
            "gaussian_random_field", "corner_coords", "WEST", "EAST", "SOUTH", "NORTH"]

@@ -1,0 +1,128 @@
+"""Serving loop: continuous batching of scenario requests on one GPU.
+
+Requests (a scenario and a number of steps) arrive over time.  The server keeps every
+admitted scenario resident in one Batch and advances all of them together in blocks of
+steps; between blocks it admits arrived requests while their device footprint fits the
+HBM budget (first come, first served) and retires finished ones, so requests that arrive
+while others run join the running batch.  A request's latency is its completion time
+minus its arrival time.
+
+The comparison point is the plain engine serving the same stream one request at a time
+(first come, first served): each request's run time alone is measured on the GPU and the
+queue is replayed from those times.
+"""
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import Batch, ShallowWaterEngine, estimate_device_bytes
+
+HBM_BYTES = 288 * 1024 ** 3       # MI355X HBM3E capacity
+
+
+@dataclass
+class Record:
+    rid: int
+    cells: int
+    steps: int
+    arrival: float
+    admitted: float = 0.0
+    finished: float = 0.0
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def summarize(records, makespan):
+    lat = np.array([r.latency for r in records])
+    work = sum(r.cells * r.steps for r in records)
+    return {"requests": len(records), "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
+            "makespan_s": makespan, "gcell_steps_per_s": work / makespan / 1e9, "cell_steps": work}
+
+
+def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=False, clock=time.perf_counter,
+          sleep=time.sleep):
+    """Runs the request stream in real time through one Batch.  Returns (records,
+    summary, states) where states maps request id to its final state (FP64 view of the
+    state, if keep_states)."""
+    pending = deque(sorted(requests, key=lambda r: r.arrival))
+    queue = deque()
+    active = {}                    # batch id -> [request, remaining steps, bytes, record]
+    records = []
+    states = {}
+    used = 0
+    b = Batch(graph_steps=block)
+    t0 = clock()
+    try:
+        while pending or queue or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            # admission: FIFO while the footprint fits the budget
+            while queue and len(active) < max_active:
+                req = queue[0]
+                need = estimate_device_bytes(req.scenario.s, req.scenario.boundaries)
+                if used + need > hbm_budget and active:
+                    break
+                queue.popleft()
+                sc = req.scenario
+                sid = b.add(sc.s, sc.boundaries, sc.bottom, sc.state)
+                rec = Record(req.rid, sc.s.nx * sc.s.ny, req.steps, req.arrival, admitted=clock() - t0)
+                active[sid] = [req, req.steps, need, rec]
+                used += need
+            if not active:
+                if pending:
+                    sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            k = min(block, min(v[1] for v in active.values()))
+            b.step(k)
+            b.sync()
+            done_at = clock() - t0
+            for sid in list(active):
+                v = active[sid]
+                v[1] -= k
+                if v[1] == 0:
+                    v[3].finished = done_at
+                    if keep_states:
+                        states[v[0].rid] = b.get_state_f64(sid)
+                    b.remove(sid)
+                    used -= v[2]
+                    records.append(v[3])
+                    del active[sid]
+    finally:
+        b.close()
+    makespan = max(r.finished for r in records) - min(r.arrival for r in records)
+    return records, summarize(records, makespan), states
+
+
+def alone_run_time(req, graph_steps=0):
+    """Device time of one request run alone with the plain engine (seconds)."""
+    sc = req.scenario
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    if graph_steps:
+        e.set_graph_steps(graph_steps)
+    t = time.perf_counter()
+    e.timestep(req.steps)
+    e.sync()
+    dt = time.perf_counter() - t
+    e.close()
+    return dt
+
+
+def one_at_a_time(requests, graph_steps=0):
+    """FIFO replay of the stream on the plain engine: each request's measured run time alone,
+    queued behind the previous one."""
+    recs = []
+    free_at = 0.0
+    for req in sorted(requests, key=lambda r: r.arrival):
+        run = alone_run_time(req, graph_steps)
+        start = max(req.arrival, free_at)
+        free_at = start + run
+        sc = req.scenario
+        recs.append(Record(req.rid, sc.s.nx * sc.s.ny, req.steps, req.arrival, admitted=start, finished=free_at))
+    makespan = max(r.finished for r in recs) - min(r.arrival for r in recs)
+    return recs, summarize(recs, makespan)

@@ -1,0 +1,80 @@
+"""Serving loop: continuous batching of a request stream (celeris_amd.serving).
+
+GPU: every request's result is bit-identical alone and in the mix (the "alone vs mixed"
+check), also when the HBM budget forces requests to wait for admission; the one-at-a-time
+replay is a FIFO queue over measured run times.  CPU: the request generator's
+distributions and the latency summary.
+"""
+import numpy as np
+import pytest
+
+from celeris_amd import ShallowWaterEngine, estimate_device_bytes, serving
+from celeris_amd import scenarios as S
+
+
+def small_stream(n=6, rate=200.0, seed=3):
+    return S.serving_requests(n, rate, seed=seed, sides=(48, 64, 96, 128), steps_range=(20, 120))
+
+
+def test_request_generator_distributions():
+    reqs = S.serving_requests(300, 20.0, seed=1, sides=(16, 24, 32, 40), steps_range=(500, 20000))
+    arr = np.array([r.arrival for r in reqs])
+    gaps = np.diff(np.concatenate([[0.0], arr]))
+    assert abs(gaps.mean() - 1 / 20.0) < 0.2 / 20.0          # Poisson process of rate 20/s
+    sides = np.array([r.scenario.s.nx for r in reqs])
+    frac = [np.mean(sides == s) for s in (16, 24, 32, 40)]
+    assert np.allclose(frac, [0.4, 0.3, 0.2, 0.1], atol=0.07)
+    steps = np.array([r.steps for r in reqs])
+    assert steps.min() >= 500 and steps.max() <= 20000
+    lg = np.log(steps)
+    assert abs(lg.mean() - 0.5 * (np.log(500) + np.log(20000))) < 0.25   # log-uniform
+
+
+def test_summary_percentiles():
+    recs = [serving.Record(i, 100, 10, arrival=0.0, finished=float(i + 1)) for i in range(100)]
+    s = serving.summarize(recs, makespan=100.0)
+    assert s["p50_latency_s"] == pytest.approx(50.5)
+    assert s["p99_latency_s"] == pytest.approx(99.01)
+    assert s["gcell_steps_per_s"] == pytest.approx(100 * 100 * 10 / 100.0 / 1e9)
+
+
+def alone_state(req):
+    sc = req.scenario
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(req.steps)
+    st = e.get_state_f64()
+    e.close()
+    return st
+
+
+@pytest.mark.gpu
+def test_served_results_equal_alone():
+    reqs = small_stream()
+    recs, summ, states = serving.serve(reqs, block=16, keep_states=True)
+    assert summ["requests"] == len(reqs)
+    for r in reqs:
+        assert np.array_equal(states[r.rid], alone_state(r)), f"request {r.rid} differs in the mix"
+    for rec in recs:
+        assert rec.finished >= rec.admitted >= rec.arrival - 1e-9
+
+
+@pytest.mark.gpu
+def test_budget_limits_admission():
+    reqs = small_stream(n=4, seed=4)
+    budget = max(estimate_device_bytes(r.scenario.s, r.scenario.boundaries) for r in reqs) + 1
+    recs, summ, states = serving.serve(reqs, hbm_budget=budget, block=16, keep_states=True)
+    recs = sorted(recs, key=lambda r: r.admitted)
+    for a, b in zip(recs, recs[1:]):
+        assert b.admitted >= a.finished - 1e-9        # one resident scenario at a time
+    for r in reqs:
+        assert np.array_equal(states[r.rid], alone_state(r))
+
+
+@pytest.mark.gpu
+def test_one_at_a_time_is_fifo():
+    reqs = small_stream(n=4, seed=5)
+    recs, summ = serving.one_at_a_time(reqs)
+    recs = sorted(recs, key=lambda r: r.arrival)
+    for a, b in zip(recs, recs[1:]):
+        assert b.admitted == pytest.approx(max(b.arrival, a.finished))
+    assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
