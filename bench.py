@@ -53,22 +53,29 @@ def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
                       f"{dt:.1f} s, OpenMP {threads} threads"}
 
 
-def serving_leg(n, rate, seed, block):
+def serving_leg(n, rates, seed, block):
     """A bounded sample of the serving stream: n requests (generated before the clock
-    starts), served in real time with continuous batching, then replayed one at a time."""
+    starts), served in real time with continuous batching at each arrival rate, and the
+    plain engine serving them one at a time (FIFO replay of each request's measured run
+    time alone).  The same seed gives the same requests at every rate; arrival times scale
+    with 1/rate."""
     from celeris_amd import scenarios as S
     from celeris_amd import serving
-    reqs = S.serving_requests(n, rate, seed=seed)
-    _, batched, _ = serving.serve(reqs, block=block)
-    _, alone = serving.one_at_a_time(reqs)
+    base = S.serving_requests(n, 1.0, seed=seed)
+    alone_times = {r.rid: serving.alone_run_time(r) for r in base}
     sides = {}
-    for r in reqs:
+    for r in base:
         sides[r.scenario.s.nx] = sides.get(r.scenario.s.nx, 0) + 1
-    return {"workload": f"{n} requests, Poisson arrivals at {rate}/s, square grids {dict(sorted(sides.items()))}, "
-                        "500-20000 steps (log-uniform), sloped seabed + GRF, sine/irregular forcing, "
-                        f"compensated update, {block}-step blocks",
-            "batched": batched, "one_at_a_time": alone,
-            "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
+    out = {"workload": f"{n} requests, square grids {dict(sorted(sides.items()))}, 500-20000 steps (log-uniform), "
+                       f"sloped seabed + GRF, sine/irregular forcing, compensated update, {block}-step blocks",
+           "rates": {}}
+    for rate in rates:
+        reqs = [S.Request(r.rid, r.arrival / rate, r.scenario, r.steps) for r in base]
+        _, batched, _ = serving.serve(reqs, block=block)
+        _, alone = serving.one_at_a_time(reqs, run_times=alone_times)
+        out["rates"][f"{rate:g}/s"] = {"batched": batched, "one_at_a_time": alone,
+                                       "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
+    return out
 
 
 def ensemble_line(args):
@@ -119,26 +126,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--graph-steps", type=int, default=50,
-                    help="steps per captured hipGraph block (0: launch step by step)")
+    ap.add_argument("--graph-steps", type=int, default=0,
+                    help="steps per captured hipGraph block (0: launch step by step, the faster mode measured on "
+                         "MI355X; the ensemble workload defaults to 50-step blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["headline", "ensemble"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
-    ap.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serving-rates", type=str, default="5,50", help="Poisson arrival rates (1/s), comma-separated")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy a 256 MiB device buffer 10 times (reference for HBM byte counters)")
+                    help="only copy a 1 GiB device buffer 4 times (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
+        # 1 GiB source and destination: four times the 256 MB last-level cache, so every
+        # copy reads and writes HBM
         import torch
-        n = 64 * 1024 * 1024                      # 256 MiB of float32
+        n = 256 * 1024 * 1024
         a = torch.ones(n, device="cuda:0")
         b = torch.empty_like(a)
-        for _ in range(10):
+        for _ in range(4):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 10}))
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 4}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -146,6 +156,8 @@ def main():
         sys.exit(2)
 
     if args.workload == "ensemble":
+        if args.graph_steps == 0 and "--graph-steps" not in sys.argv:
+            args.graph_steps = 50
         print(json.dumps(ensemble_line(args)), flush=True)
         return
 
@@ -203,8 +215,11 @@ def main():
                      "bytes_per_launch": BYTES_PER_CELL_STEP * cells},
         "cpu_baseline": None,
     }
+    print(f"bench: 2048^2 step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
     if args.serving_requests > 0:
-        out["serving"] = serving_leg(args.serving_requests, args.serving_rate, 5, args.graph_steps or 50)
+        rates = [float(v) for v in args.serving_rates.split(",")]
+        out["serving"] = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)
+        print("bench: serving sample done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

@@ -187,9 +187,10 @@ __device__ __forceinline__ void line_abc(float K1, float Q, bool first, bool las
 struct CellIn {
     float o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
     float Kx, Qx, Ky, Qy, ddx, ddy;
+    float wlo;      // low word of w (compensated state)
 };
 
-template <int SCHEME>
+template <int SCHEME, bool COMP>
 __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
 {
     CellIn c;
@@ -204,6 +205,7 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
     c.Kx = ld32(p.Kx, i - 2u); c.Qx = ld32(p.Qx, i - 2u);
     c.Ky = ld32(p.Ky, i - 2u); c.Qy = ld32(p.Qy, i - 2u);
     c.ddx = ld32(p.DDx, i - 2u); c.ddy = ld32(p.DDy, i - 2u);
+    c.wlo = COMP ? ld32(p.w_lo, i) : 0.f;
     return c;
 }
 
@@ -304,7 +306,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     const int ylast = p.ny + 3;
     float nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
     float nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys);
-    CellIn nx_c = load_cell<SCHEME>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
+    CellIn nx_c = load_cell<SCHEME, COMP>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
 
     for (int y = ys; y < ye; ++y) {
         // ---- rows y+2 (w, hu, hv, BY) and y (BX, BA, per-cell inputs), prefetched
@@ -322,7 +324,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             nx_BY = ldrow(p.BY, yn);
             nx_BX = ldrow(p.BX, yc);
             nx_BA = ldrow(p.BA, yc);
-            nx_c = load_cell<SCHEME>(p, (unsigned)yc * (unsigned)P + (unsigned)xc);
+            nx_c = load_cell<SCHEME, COMP>(p, (unsigned)yc * (unsigned)P + (unsigned)xc);
         }
 
         // ---- faces of row y+1 along y: S face used now, N face kept
@@ -484,7 +486,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             if (COMP) {
                 // (w_hi, w_lo) += dw as a two-float sum; the lines solve for the increments
                 float lo;
-                two_sum(wc, dw + ld32(p.w_lo, i), wOut, lo);
+                two_sum(wc, dw + cin.wlo, wOut, lo);
                 st32(p.w_lo_new, i, lo);
                 huOut = dhu;
                 hvOut = dhv;
@@ -623,8 +625,12 @@ __device__ __forceinline__ Eq reduce_eq(const Eq &L, const Eq &H, const Eq &R)
 
 template <int C, int W>
 struct LineSmem {
-    Eq top[64 * W];                            // one unknown per thread above the chunk levels
+    // one unknown per thread above the chunk levels, as four planes: level j touches every
+    // 2^j-th thread, which a 16-byte record per thread would map onto a quarter of the banks
+    float ta[64 * W], tb[64 * W], tc[64 * W], td[64 * W];
     Eq xch[2][64 * W];                         // chunk-level neighbour exchange (double-buffered)
+    __device__ Eq top(int i) const { return Eq{ta[i], tb[i], tc[i], td[i]}; }
+    __device__ void set_top(int i, const Eq &e) { ta[i] = e.a; tb[i] = e.b; tc[i] = e.c; td[i] = e.d; }
 };
 
 // Loads C consecutive floats starting at a 16-byte aligned element (C % 4 == 0) or singly.
@@ -697,15 +703,15 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
     }
     // ---- reduction across threads: levels K0+1 .. K0+KT, in LDS
     const int mypos = base + C - 1;
-    sm.top[t] = q[C - 1];
+    sm.set_top(t, q[C - 1]);
     __syncthreads();
 #pragma unroll
     for (int j = 1; j <= KT; ++j) {
         const int s = 1 << (j - 1);
         if (((t + 1) & ((1 << j) - 1)) == 0 && mypos < n) {
-            const Eq L = sm.top[t - s];
-            const Eq R = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top[t + s] : margin;
-            sm.top[t] = reduce_eq(L, sm.top[t], R);
+            const Eq L = sm.top(t - s);
+            const Eq R = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top(t + s) : margin;
+            sm.set_top(t, reduce_eq(L, sm.top(t), R));
         }
         __syncthreads();
     }
@@ -714,15 +720,15 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
     for (int j = KT; j >= 0; --j) {
         const int s = 1 << j;
         if (((t + 1) & ((2 << j) - 1)) == s && mypos < n) {
-            const float xl = t - s >= 0 ? sm.top[t - s].d : 0.f;
-            const float xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top[t + s].d : 0.f;
-            Eq e = sm.top[t];
-            sm.top[t].d = (e.d - e.a * xl - e.c * xr) / e.b;
+            const float xl = t - s >= 0 ? sm.td[t - s] : 0.f;
+            const float xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.td[t + s] : 0.f;
+            const Eq e = sm.top(t);
+            sm.td[t] = (e.d - e.a * xl - e.c * xr) / e.b;
         }
         __syncthreads();
     }
-    q[C - 1].d = sm.top[t].d;
-    const float prev_last = t > 0 ? sm.top[t - 1].d : 0.f;
+    q[C - 1].d = sm.td[t];
+    const float prev_last = t > 0 ? sm.td[t - 1] : 0.f;
     // ---- back-substitution inside the chunk: levels K0-1 .. 0
 #pragma unroll
     for (int k = K0 - 1; k >= 0; --k) {
@@ -736,7 +742,18 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
-    if (comp) {
+    if (comp && C % 2 == 0 && base + C <= n) {
+#pragma unroll
+        for (int i = 0; i < C; i += 2) {
+            const float2 pv = *reinterpret_cast<const float2 *>(prev + base + i);
+            const float2 pl = *reinterpret_cast<const float2 *>(prev_lo + base + i);
+            float h0, l0, h1, l1;
+            two_sum(pv.x, q[i].d + pl.x, h0, l0);
+            two_sum(pv.y, q[i + 1].d + pl.y, h1, l1);
+            *reinterpret_cast<float2 *>(out + base + i) = make_float2(h0, h1);
+            *reinterpret_cast<float2 *>(out_lo + base + i) = make_float2(l0, l1);
+        }
+    } else if (comp) {
 #pragma unroll
         for (int i = 0; i < C; ++i)
             if (base + i < n) {

@@ -44,9 +44,11 @@ def summarize(records, makespan):
             "makespan_s": makespan, "gcell_steps_per_s": work / makespan / 1e9, "cell_steps": work}
 
 
-def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=False, clock=time.perf_counter,
-          sleep=time.sleep):
-    """Runs the request stream in real time through one Batch.  Returns (records,
+def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=False, graph_steps=0,
+          clock=time.perf_counter, sleep=time.sleep):
+    """Runs the request stream in real time through one Batch, synchronising every `block`
+    steps (or sooner, when a request completes).  graph_steps > 0 replays captured blocks
+    of that many steps (on MI355X direct launches measured faster).  Returns (records,
     summary, states) where states maps request id to its final state (FP64 view of the
     state, if keep_states)."""
     pending = deque(sorted(requests, key=lambda r: r.arrival))
@@ -55,7 +57,7 @@ def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=
     records = []
     states = {}
     used = 0
-    b = Batch(graph_steps=block)
+    b = Batch(graph_steps=graph_steps)
     t0 = clock()
     try:
         while pending or queue or active:
@@ -113,13 +115,13 @@ def alone_run_time(req, graph_steps=0):
     return dt
 
 
-def one_at_a_time(requests, graph_steps=0):
-    """FIFO replay of the stream on the plain engine: each request's measured run time alone,
-    queued behind the previous one."""
+def one_at_a_time(requests, graph_steps=0, run_times=None):
+    """FIFO replay of the stream on the plain engine: each request's measured run time alone
+    (or run_times[rid] if given, measured the same way), queued behind the previous one."""
     recs = []
     free_at = 0.0
     for req in sorted(requests, key=lambda r: r.arrival):
-        run = alone_run_time(req, graph_steps)
+        run = run_times[req.rid] if run_times is not None else alone_run_time(req, graph_steps)
         start = max(req.arrival, free_at)
         free_at = start + run
         sc = req.scenario

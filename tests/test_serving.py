@@ -50,7 +50,7 @@ def alone_state(req):
 @pytest.mark.gpu
 def test_served_results_equal_alone():
     reqs = small_stream()
-    recs, summ, states = serving.serve(reqs, block=16, keep_states=True)
+    recs, summ, states = serving.serve(reqs, block=16, keep_states=True, graph_steps=16)
     assert summ["requests"] == len(reqs)
     for r in reqs:
         assert np.array_equal(states[r.rid], alone_state(r)), f"request {r.rid} differs in the mix"
