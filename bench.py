@@ -31,6 +31,17 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_CELL_STEP = 96          # algorithmic HBM bytes per interior cell-step (plain FP32)
 HBM_PEAK_GBS = 8000.0             # MI355X HBM3E peak
+# measured HBM bytes per explicit_step launch at 2048^2 (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# scaled by the copy calibration) and the same kernel's VALU issue utilisation
+PMC_FILE = os.path.join(ROOT, "profiles", "explicit_step_2048x2048_pmc.json")
+
+
+def measured_traffic(nx, ny):
+    if (nx, ny) != (2048, 2048) or not os.path.exists(PMC_FILE):
+        return None, None
+    with open(PMC_FILE) as f:
+        d = json.load(f)
+    return d["traffic_bytes_per_launch"], d.get("valu_busy")
 
 
 def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
@@ -141,14 +152,18 @@ def main():
     if args.copy_calibration:
         # 1 GiB source and destination: four times the 256 MB last-level cache, so every
         # copy reads and writes HBM
+        # (copy: 1 GiB written per launch; sum: 1 GiB read per launch)
         import torch
         n = 256 * 1024 * 1024
         a = torch.ones(n, device="cuda:0")
         b = torch.empty_like(a)
         for _ in range(4):
             b.copy_(a)
+        s = 0.0
+        for _ in range(4):
+            s += float(b.sum())
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 4}))
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "sum_bytes_per_launch": 4 * n, "launches": 4, "check": s}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -191,6 +206,7 @@ def main():
 
     avg_ms = kern["explicit_step"]
     achieved = BYTES_PER_CELL_STEP * cells / (avg_ms * 1e-3) / 1e9
+    traffic, valu_busy = measured_traffic(sc.s.nx, sc.s.ny)
     out = {
         "metric": "Gcell-steps/s (2048^2 Boussinesq)",
         "value": value,
@@ -211,8 +227,10 @@ def main():
                    "event_ms_per_step": ms_events / args.steps,
                    "kernel_avg_ms": kern, "state_finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "explicit_step",
-                     "bytes_per_launch": BYTES_PER_CELL_STEP * cells},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "explicit_step",
+                     "bytes_per_launch": BYTES_PER_CELL_STEP * cells,
+                     "measured_hbm_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9,
+                     "valu_busy": valu_busy},
         "cpu_baseline": None,
     }
     print(f"bench: 2048^2 step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
