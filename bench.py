@@ -96,7 +96,9 @@ def ensemble_line(args):
     cells = sum(m.s.nx * m.s.ny for m in members)
     b = Batch(graph_steps=args.graph_steps)
     ids = [b.add(m.s, m.boundaries, m.bottom, m.state) for m in members]
-    b.step(args.warmup)
+    b.step(min(args.warmup, 10))
+    band_rows = b.autotune(args.autotune, tune_lines=True) if args.autotune > 0 else None
+    b.step(args.warmup - min(args.warmup, 10))
     b.sync()
     t0 = time.perf_counter()
     ms = b.timed_steps(args.steps)
@@ -124,6 +126,7 @@ def ensemble_line(args):
                                f"({sum(nx * ny for nx, ny in sides)} cells), dx 0.5 m, sloped seabed + GRF, "
                                "sine/irregular forcing, compensated update",
                    "graph_steps": args.graph_steps, "event_ms_per_step": ms / args.steps,
+                   "autotuned_band_rows": band_rows,
                    "one_at_a_time_gcell_steps_per_s": alone, "speedup_vs_one_at_a_time": batched / alone,
                    "state_finite": finite, "smallest": min(sides, key=lambda v: v[0] * v[1]),
                    "largest": max(sides, key=lambda v: v[0] * v[1])},
@@ -140,6 +143,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="steps per captured hipGraph block (0: launch step by step, the faster mode measured on "
                          "MI355X; the ensemble workload defaults to 50-step blocks)")
+    ap.add_argument("--autotune", type=int, default=10,
+                    help="repeats per candidate when autotuning band height and line-solve shapes (0: defaults)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["headline", "ensemble"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
@@ -183,7 +188,9 @@ def main():
     eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     cells = sc.s.nx * sc.s.ny
     eng.set_graph_steps(args.graph_steps)
-    eng.timestep(args.warmup)
+    eng.timestep(min(args.warmup, 10))
+    tuning = eng.autotune(args.autotune) if args.autotune > 0 else eng.tuning()
+    eng.timestep(args.warmup - min(args.warmup, 10))
     eng.sync()
 
     # timed region: K steps bracketed by device synchronisation, wall clock and HIP events
@@ -223,6 +230,7 @@ def main():
         "config": {"workload": f"Boussinesq step, {sc.s.nx}x{sc.s.ny} interior cells, dx 1 m, 1:30 slope + GRF, "
                                "west irregular waves (100 JONSWAP components), 20-cell east sponge, N/S walls, FP32",
                    "nx": sc.s.nx, "ny": sc.s.ny, "dt": sc.s.timestep, "graph_steps": args.graph_steps,
+                   "tuning": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in tuning.items()},
                    "hbm_frac_step": value * 1e9 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
                    "event_ms_per_step": ms_events / args.steps,
                    "kernel_avg_ms": kern, "state_finite": finite},

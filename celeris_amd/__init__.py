@@ -55,6 +55,15 @@ class CelAdaptive(ctypes.Structure):
     _fields_ = [("ema_alpha", ctypes.c_float), ("min_timestep_ratio", ctypes.c_float), ("target_cfl", ctypes.c_float)]
 
 
+class CelTuning(ctypes.Structure):
+    _fields_ = [("band_rows", ctypes.c_int), ("x_chunk", ctypes.c_int), ("x_waves", ctypes.c_int),
+                ("y_chunk", ctypes.c_int), ("y_waves", ctypes.c_int), ("explicit_us", ctypes.c_float),
+                ("x_lines_us", ctypes.c_float), ("y_lines_us", ctypes.c_float)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # Every symbol include/celeris.h declares, with its argument types.
 SIGNATURES = {
     "cel_last_error": ([], ctypes.c_char_p),
@@ -79,6 +88,10 @@ SIGNATURES = {
     "cel_engine_coefficients": ([ctypes.c_void_p, _F, _F], ctypes.c_int),
     "cel_estimate_device_bytes": ([ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary),
                                    ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "cel_engine_autotune": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CelTuning)], ctypes.c_int),
+    "cel_engine_get_tuning": ([ctypes.c_void_p, ctypes.POINTER(CelTuning)], ctypes.c_int),
+    "cel_engine_set_tuning": ([ctypes.c_void_p, ctypes.POINTER(CelTuning)], ctypes.c_int),
+    "cel_batch_autotune": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_batch_create": ([ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "cel_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "cel_batch_add": ([ctypes.c_void_p, ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
@@ -269,6 +282,22 @@ class ShallowWaterEngine:
         _check(load_library().cel_engine_device_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def autotune(self, repeats=10):
+        """Times the band heights and line-solve block shapes on the current state and keeps
+        the fastest (results are unchanged); returns the choice as a dict."""
+        t = CelTuning()
+        _check(load_library().cel_engine_autotune(self._h, repeats, ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = CelTuning()
+        _check(load_library().cel_engine_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, band_rows, x_chunk, x_waves, y_chunk, y_waves):
+        t = CelTuning(band_rows, x_chunk, x_waves, y_chunk, y_waves, 0.0, 0.0, 0.0)
+        _check(load_library().cel_engine_set_tuning(self._h, ctypes.byref(t)))
+
     def set_graph_steps(self, k):
         """Run cel_engine_step in captured blocks of k steps (0: step by step)."""
         _check(load_library().cel_engine_set_graph_steps(self._h, k))
@@ -345,6 +374,12 @@ class Batch:
 
     def step(self, n=1):
         _check(load_library().cel_batch_step(self._h, n))
+
+    def autotune(self, repeats=10, tune_lines=True):
+        """Band height of the batched explicit launch (and each scenario's line-solve shapes)."""
+        r = ctypes.c_int()
+        _check(load_library().cel_batch_autotune(self._h, repeats, int(bool(tune_lines)), ctypes.byref(r)))
+        return r.value
 
     def set_graph_steps(self, k):
         _check(load_library().cel_batch_set_graph_steps(self._h, k))

@@ -114,40 +114,45 @@ void Batch::ensureBlob(size_t bytes)
     blob_used_[0] = blob_used_[1] = false;
 }
 
-// Work counts of each scenario (from its first planned step; they depend on the grid and
-// boundary configuration only), line-length classes, start offsets of every work-list.
+// Work counts of each scenario (from its first planned step; they depend on the grid,
+// the boundary configuration and the chosen shapes only), start offsets of every
+// work-list.  The line solves are listed per line direction (x-lines carry the
+// irregular-wave lines), grouped by block shape.
 void Batch::buildComposition(const std::vector<StepLaunch> &first, const std::vector<int> &ids)
 {
     const int ns = (int)ids.size();
     Composition c;
     c.ids = ids;
-    std::vector<int> tiles(ns), blocks(ns), cells(ns), cls(ns);
+    std::vector<int> tiles(ns), cells(ns), shp(2 * ns), blocks(2 * ns);
     for (int i = 0; i < ns; ++i) {
         const StepLaunch &L = first[i];
         tiles[i] = L.ep.nstrips * L.ep.nbands;
-        cls[i] = solve_class(std::max(L.lp.nx, L.lp.ny));
+        shp[2 * i] = L.shape_x;
+        shp[2 * i + 1] = L.shape_y;
         int irr_lines = 0;
         for (int s = 0; s < 4; ++s) irr_lines += L.irr.count[s];
-        const int W = solve_class_waves(cls[i]);
-        blocks[i] = L.lp.nlines_x + L.lp.nlines_y + (irr_lines + W - 1) / W;
+        const int W = kShapeW[L.shape_x];
+        blocks[2 * i] = L.lp.nlines_x + (irr_lines + W - 1) / W;
+        blocks[2 * i + 1] = L.lp.nlines_y;
         cells[i] = L.bp.ncells;
-        c.sig.insert(c.sig.end(), {ids[i], tiles[i], cls[i], blocks[i], cells[i]});
+        c.sig.insert(c.sig.end(), {ids[i], tiles[i], shp[2 * i], shp[2 * i + 1], blocks[2 * i], blocks[2 * i + 1],
+                                   cells[i]});
     }
-    c.solve_order.resize(ns);
-    for (int i = 0; i < ns; ++i) c.solve_order[i] = i;
-    std::stable_sort(c.solve_order.begin(), c.solve_order.end(), [&](int a, int b) { return cls[a] < cls[b]; });
-    // starts buffer: explicit (ns+1), per class (n_c+1), boundary (ns+1)
+    c.solve_order.resize(2 * ns);
+    for (int i = 0; i < 2 * ns; ++i) c.solve_order[i] = i;
+    std::stable_sort(c.solve_order.begin(), c.solve_order.end(), [&](int a, int b) { return shp[a] < shp[b]; });
+    // starts buffer: explicit (ns+1), per shape (n_c+1), boundary (ns+1)
     std::vector<int> st;
     c.exp_starts_off = 0;
     st.push_back(0);
     for (int i = 0; i < ns; ++i) st.push_back(st.back() + tiles[i]);
     c.total_tiles = st.back();
     int pos = 0;
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < kNumShapes; ++k) {
         c.cls_first[k] = pos;
         c.cls_starts_off[k] = (int)st.size();
         st.push_back(0);
-        while (pos < ns && cls[c.solve_order[pos]] == k) {
+        while (pos < 2 * ns && shp[c.solve_order[pos]] == k) {
             st.push_back(st.back() + blocks[c.solve_order[pos]]);
             ++pos;
         }
@@ -170,7 +175,7 @@ void Batch::buildComposition(const std::vector<StepLaunch> &first, const std::ve
     check(hipMemcpy(d_starts_, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice), "upload batch lists");
     sec_exp_ = 0;
     sec_solve_ = align256((size_t)ns * sizeof(ExplicitJob));
-    sec_bnd_ = sec_solve_ + align256((size_t)ns * sizeof(SolveJob));
+    sec_bnd_ = sec_solve_ + align256((size_t)2 * ns * sizeof(SolveJob));
     stride_ = sec_bnd_ + align256((size_t)ns * sizeof(BoundaryJob));
     comp_ = std::move(c);
     comp_valid_ = true;
@@ -185,9 +190,9 @@ void Batch::issue(const char *blob, size_t stride, int m)
         const ExplicitJob *ej = reinterpret_cast<const ExplicitJob *>(b + sec_exp_);
         const SolveJob *sj = reinterpret_cast<const SolveJob *>(b + sec_solve_);
         const BoundaryJob *bj = reinterpret_cast<const BoundaryJob *>(b + sec_bnd_);
-        check(launch_explicit_batched(ej, d_starts_ + comp_.exp_starts_off, ns, comp_.total_tiles, stream_),
+        check(launch_explicit_batched(rows_, ej, d_starts_ + comp_.exp_starts_off, ns, comp_.total_tiles, stream_),
               "explicit_batched");
-        for (int c = 0; c < 8; ++c)
+        for (int c = 0; c < kNumShapes; ++c)
             if (comp_.cls_count[c] > 0)
                 check(launch_solve_batched(c, sj + comp_.cls_first[c], d_starts_ + comp_.cls_starts_off[c],
                                            comp_.cls_count[c], comp_.cls_blocks[c], stream_), "solve_batched");
@@ -197,41 +202,44 @@ void Batch::issue(const char *blob, size_t stride, int m)
     }
 }
 
-void Batch::run(int m, bool use_graph)
+void Batch::prepare(int m, bool advance, std::vector<StepLaunch> &plans, std::vector<ShallowWaterEngine *> &eng)
 {
     const int ns = (int)scen_.size();
-    if (ns == 0 || m == 0) return;
     std::vector<int> ids;
-    std::vector<ShallowWaterEngine *> eng;
+    eng.clear();
     for (auto &kv : scen_) { ids.push_back(kv.first); eng.push_back(kv.second.get()); }
-    // plan m steps of every scenario
-    std::vector<StepLaunch> plans((size_t)m * ns);
+    plans.assign((size_t)m * ns, StepLaunch{});
     for (int k = 0; k < m; ++k)
-        for (int i = 0; i < ns; ++i) eng[i]->planStep(plans[(size_t)k * ns + i]);
+        for (int i = 0; i < ns; ++i) {
+            StepLaunch &L = plans[(size_t)k * ns + i];
+            eng[i]->planStep(L, advance);
+            L.ep.nbands = (L.ep.ny + rows_ - 1) / rows_;      // the batch's band height
+        }
     for (const auto &L : plans)
         if (L.scheme == CEL_CORRECTOR || L.ncopy > 0) throw Error(CEL_ESTATE, "corrector step in a batch");
-    {
-        std::vector<StepLaunch> first(plans.begin(), plans.begin() + ns);
-        std::vector<int> sig;
-        if (comp_valid_) {
-            // recompute the signature cheaply and compare
-            for (int i = 0; i < ns; ++i) {
-                const StepLaunch &L = first[i];
-                const int cls = solve_class(std::max(L.lp.nx, L.lp.ny));
-                int irr_lines = 0;
-                for (int s = 0; s < 4; ++s) irr_lines += L.irr.count[s];
-                const int W = solve_class_waves(cls);
-                sig.insert(sig.end(), {ids[i], L.ep.nstrips * L.ep.nbands, cls,
-                                       L.lp.nlines_x + L.lp.nlines_y + (irr_lines + W - 1) / W, L.bp.ncells});
-            }
+    std::vector<int> sig;
+    if (comp_valid_)
+        for (int i = 0; i < ns; ++i) {
+            const StepLaunch &L = plans[i];
+            int irr_lines = 0;
+            for (int s = 0; s < 4; ++s) irr_lines += L.irr.count[s];
+            const int W = kShapeW[L.shape_x];
+            sig.insert(sig.end(), {ids[i], L.ep.nstrips * L.ep.nbands, L.shape_x, L.shape_y,
+                                   L.lp.nlines_x + (irr_lines + W - 1) / W, L.lp.nlines_y, L.bp.ncells});
         }
-        if (!comp_valid_ || sig != comp_.sig) buildComposition(first, ids);
+    if (!comp_valid_ || sig != comp_.sig) {
+        std::vector<StepLaunch> first(plans.begin(), plans.begin() + ns);
+        buildComposition(first, ids);
     }
     ensureBlob(stride_ * (size_t)std::max(m, std::max(graph_steps_, 1)));
-    // fill the host table: per step, explicit jobs in scenario order, solve jobs in class
-    // order, boundary jobs in scenario order
-    char *h = h_blob_[blob_slot_];
-    if (blob_used_[blob_slot_]) check(hipEventSynchronize(blob_ev_[blob_slot_]), "table event");
+}
+
+// Host table: per step, explicit jobs in scenario order, solve jobs in shape order,
+// boundary jobs in scenario order.
+void Batch::fillTable(const std::vector<StepLaunch> &plans, const std::vector<ShallowWaterEngine *> &eng, int m,
+                      char *h)
+{
+    const int ns = (int)eng.size();
     for (int k = 0; k < m; ++k) {
         char *b = h + (size_t)k * stride_;
         ExplicitJob *ej = reinterpret_cast<ExplicitJob *>(b + sec_exp_);
@@ -245,15 +253,32 @@ void Batch::run(int m, bool use_graph)
             bj[i].p = L.bp;
             bj[i].p.clock = nullptr;
         }
-        for (int q = 0; q < ns; ++q) {
-            const int i = comp_.solve_order[q];
+        for (int q = 0; q < 2 * ns; ++q) {
+            const int e = comp_.solve_order[q], i = e / 2;
             const StepLaunch &L = plans[(size_t)k * ns + i];
             sj[q].p = L.lp;
             sj[q].irr = L.irr;
             sj[q].irr.clock = nullptr;
+            if (e % 2 == 0) {
+                sj[q].p.nlines_y = 0;                             // x-lines and irregular-wave lines
+            } else {
+                sj[q].p.nlines_x = 0;                             // y-lines
+                for (int s = 0; s < 4; ++s) sj[q].irr.count[s] = 0;
+            }
             sj[q].comp = eng[i]->compensated();
         }
     }
+}
+
+void Batch::run(int m, bool use_graph)
+{
+    if (scen_.empty() || m == 0) return;
+    std::vector<StepLaunch> plans;
+    std::vector<ShallowWaterEngine *> eng;
+    prepare(m, true, plans, eng);
+    char *h = h_blob_[blob_slot_];
+    if (blob_used_[blob_slot_]) check(hipEventSynchronize(blob_ev_[blob_slot_]), "table event");
+    fillTable(plans, eng, m, h);
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
         for (const auto &g : graphs_)
@@ -281,6 +306,47 @@ void Batch::run(int m, bool use_graph)
     blob_slot_ ^= 1;
     if (exec) check(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
     else issue(d_blob_, stride_, m);
+}
+
+// Band height of the batched explicit launch: each candidate timed on the current step's
+// parameters (the launch writes only buffers the next step rewrites).
+int Batch::autotune(int repeats, bool tune_lines)
+{
+    if (repeats < 1 || repeats > 1000) throw Error(CEL_EINVAL, "repeats must lie in [1, 1000]");
+    sync();
+    if (tune_lines)
+        for (auto &kv : scen_) kv.second->autotune(repeats, nullptr);
+    if (scen_.empty()) return rows_;
+    float best = -1.f;
+    int best_rows = rows_;
+    for (int rows : {4, 8, 16}) {
+        rows_ = rows;
+        std::vector<StepLaunch> plans;
+        std::vector<ShallowWaterEngine *> eng;
+        prepare(1, false, plans, eng);
+        sync();
+        char *h = h_blob_[0];
+        fillTable(plans, eng, 1, h);
+        check(hipMemcpy(d_blob_, h, stride_, hipMemcpyHostToDevice), "upload batch table");
+        const int ns = (int)eng.size();
+        auto launch = [&] {
+            check(launch_explicit_batched(rows_, reinterpret_cast<const ExplicitJob *>(d_blob_ + sec_exp_),
+                                          d_starts_ + comp_.exp_starts_off, ns, comp_.total_tiles, stream_),
+                  "explicit_batched");
+        };
+        launch();
+        check(hipEventRecord(ev_[0], stream_), "event");
+        for (int r = 0; r < repeats; ++r) launch();
+        check(hipEventRecord(ev_[1], stream_), "event");
+        check(hipEventSynchronize(ev_[1]), "event sync");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event");
+        if (best < 0.f || ms < best) { best = ms; best_rows = rows; }
+    }
+    rows_ = best_rows;
+    comp_valid_ = false;
+    blob_used_[0] = blob_used_[1] = false;
+    return rows_;
 }
 
 void Batch::step(int nsteps)

@@ -18,14 +18,12 @@
 
 namespace cel {
 
-hipError_t launch_explicit_batched(const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
+hipError_t launch_explicit_batched(int rows, const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
                                    hipStream_t st);
-hipError_t launch_solve_batched(int cls, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
+hipError_t launch_solve_batched(int shape, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
                                 hipStream_t st);
 hipError_t launch_boundary_batched(const BoundaryJob *jobs, const int *starts, int njobs, int total_cells,
                                    int mirror_pass, hipStream_t st);
-int solve_class(int nmax);
-int solve_class_waves(int cls);
 
 class Batch {
 public:
@@ -38,6 +36,7 @@ public:
     void remove(int id);
     void step(int nsteps);
     void setGraphSteps(int k);
+    int autotune(int repeats, bool tune_lines);
     void sync();
     float timedSteps(int nsteps);
     ShallowWaterEngine &scenario(int id);      // synchronises the batch stream first
@@ -48,14 +47,18 @@ public:
 private:
     struct Composition {
         std::vector<int> ids;                  // scenario order of the explicit/boundary lists
-        std::vector<int> solve_order;          // scenario positions sorted by line-length class
+        std::vector<int> solve_order;          // 2 i (x-lines) / 2 i + 1 (y-lines), sorted by block shape
         std::vector<int> sig;                  // per-scenario work counts and classes
         int total_tiles = 0, total_cells = 0;
-        int cls_first[8] = {}, cls_count[8] = {}, cls_blocks[8] = {};
-        int cls_starts_off[8] = {};            // offsets into the starts buffer (ints)
+        int cls_first[kNumShapes] = {}, cls_count[kNumShapes] = {}, cls_blocks[kNumShapes] = {};
+        int cls_starts_off[kNumShapes] = {};   // offsets into the starts buffer (ints)
         int exp_starts_off = 0, bnd_starts_off = 0;
     };
     void run(int m, bool use_graph);
+    // plans m steps of every scenario (m == 1 when not advancing), brings the composition
+    // and the parameter table up to date and fills the host table
+    void prepare(int m, bool advance, std::vector<StepLaunch> &plans, std::vector<ShallowWaterEngine *> &eng);
+    void fillTable(const std::vector<StepLaunch> &plans, const std::vector<ShallowWaterEngine *> &eng, int m, char *h);
     void buildComposition(const std::vector<StepLaunch> &first, const std::vector<int> &ids);
     void issue(const char *blob, size_t stride, int m);
     void clearGraphs();
@@ -66,6 +69,7 @@ private:
     std::map<int, std::unique_ptr<ShallowWaterEngine>> scen_;
     int next_id_ = 1;
     int graph_steps_ = 0;
+    int rows_ = kDefaultBandRows;              // band rows of the batched explicit launch
     Composition comp_;
     bool comp_valid_ = false;
     int *d_starts_ = nullptr;

@@ -266,6 +266,29 @@ int cel_batch_time(cel_batch *b, int id, double *t)
     });
 }
 
+int cel_engine_autotune(cel_engine *e, int repeats, cel_tuning *out)
+{
+    return guarded([&] { eng(e).autotune(repeats, out); });
+}
+
+int cel_engine_get_tuning(cel_engine *e, cel_tuning *out) { return guarded([&] { eng(e).getTuning(out); }); }
+
+int cel_engine_set_tuning(cel_engine *e, const cel_tuning *t)
+{
+    return guarded([&] {
+        if (!t) throw cel::Error(CEL_EINVAL, "null tuning");
+        eng(e).setTuning(*t);
+    });
+}
+
+int cel_batch_autotune(cel_batch *b, int repeats, int tune_lines, int *band_rows)
+{
+    return guarded([&] {
+        const int r = bat(b).autotune(repeats, tune_lines != 0);
+        if (band_rows) *band_rows = r;
+    });
+}
+
 int cel_engine_set_graph_steps(cel_engine *e, int steps) { return guarded([&] { eng(e).setGraphSteps(steps); }); }
 
 int cel_engine_stats(cel_engine *e, cel_stats *out) { return guarded([&] { eng(e).stats(out); }); }

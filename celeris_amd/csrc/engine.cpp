@@ -68,6 +68,8 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
     PT_ = (int)round_up((size_t)Y_ + 16, 64);
     Lx_ = count_of_matrices(nx_);
     Ly_ = count_of_matrices(ny_);
+    shape_x_ = default_shape(nx_);
+    shape_y_ = default_shape(ny_);
     dx_ = s.width / (float)(nx_ - 1);
     dy_ = s.length / (float)(ny_ - 1);
     PI_ = 4.0f * std::atan(1.0f);
@@ -521,7 +523,7 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     std::memset(&p, 0, sizeof(p));
     p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
     p.nstrips = (nx_ + kStripCols - 1) / kStripCols;
-    p.nbands = (ny_ + kBandRows - 1) / kBandRows;
+    p.nbands = (ny_ + rows_ - 1) / rows_;
     const int c = cur_, n = 1 - cur_;
     p.w = w_[c]; p.hu = hu_[c]; p.hvT = hvT_[c];
     p.BY = BY_; p.BX = BX_; p.BA = BA_;
@@ -598,12 +600,21 @@ void ShallowWaterEngine::fillLines(LineParams &p) const
 // Everything one step issues, from the host state at the start of the step
 // (fillConstantBuffers + the pass setup of engine.cpp:849-1438); then the host state
 // advances as after the step (engine.cpp:1476-1521).
-void ShallowWaterEngine::planStep(StepLaunch &L)
+void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
 {
     std::memset(&L, 0, sizeof(L));
     const int scheme = time_scheme_now_;
     const int n = 1 - cur_;
     L.scheme = scheme;
+    L.rows = rows_;
+    L.shape_x = shape_x_;
+    L.shape_y = shape_y_;
+    // host state the plan moves on (restored when only peeking)
+    const int keep_i[8] = {cur_, ring_old_, ring_oo_, ring_scr_, time_scheme_now_, timestep_count_,
+                           correction_steps_count_, 0};
+    const float keep_f[4] = {total_time_, current_timestep_, old_timestep_, old_old_timestep_};
+    int keep_uts[4];
+    for (int s = 0; s < 4; ++s) keep_uts[s] = side_[s].uts_iter;
     fillExplicit(L.ep);
     // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
     // steps nothing ever reads them
@@ -688,6 +699,12 @@ void ShallowWaterEngine::planStep(StepLaunch &L)
         total_time_ += current_timestep_;
     }
     cur_ = n;
+    if (!advance) {
+        cur_ = keep_i[0]; ring_old_ = keep_i[1]; ring_oo_ = keep_i[2]; ring_scr_ = keep_i[3];
+        time_scheme_now_ = keep_i[4]; timestep_count_ = keep_i[5]; correction_steps_count_ = keep_i[6];
+        total_time_ = keep_f[0]; current_timestep_ = keep_f[1]; old_timestep_ = keep_f[2]; old_old_timestep_ = keep_f[3];
+        for (int s = 0; s < 4; ++s) side_[s].uts_iter = keep_uts[s];
+    }
 }
 
 // Issues one planned step on the stream.  With dclock set, the kernels take t^n and the
@@ -699,7 +716,7 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, boo
         for (auto &e : pe) e = nextEvent();
         check(hipEventRecord(pe[0], stream_), "event");
     }
-    check(launch_explicit(L.scheme, comp_, L.ep, stream_), "explicit_step");
+    check(launch_explicit(L.scheme, comp_, L.rows, L.ep, stream_), "explicit_step");
     if (profile) check(hipEventRecord(pe[1], stream_), "event");
     for (int i = 0; i < L.ncopy; ++i)
         check(hipMemcpyAsync(L.copy[i].dst, L.copy[i].src, L.copy[i].bytes, hipMemcpyDeviceToDevice, stream_), "copy");
@@ -708,7 +725,7 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, boo
     irr.clock = dclock;
     bp.clock = dclock;
     if (profile) check(hipEventRecord(pe[2], stream_), "event");
-    check(launch_solve(comp_, L.lp, irr, stream_), "solve_lines");
+    check(launch_solve(comp_, L.shape_x, L.shape_y, L.lp, irr, stream_), "solve_lines");
     if (profile) check(hipEventRecord(pe[3], stream_), "event");
     bp.mirror_pass = 1;
     check(launch_boundary(bp, stream_), "boundary_apply");
@@ -959,6 +976,92 @@ void ShallowWaterEngine::resetTimestepAdaptive(const cel_adaptive &a, cel_stats 
     resetTimestep(cur);
     uniform_dt_ = keep_uniform;
     if (out) deriveStats(raw, out);
+}
+
+// ---- work decomposition
+
+void ShallowWaterEngine::getTuning(cel_tuning *out) const
+{
+    if (!out) throw Error(CEL_EINVAL, "null tuning");
+    out->band_rows = rows_;
+    out->x_chunk = kShapeC[shape_x_]; out->x_waves = kShapeW[shape_x_];
+    out->y_chunk = kShapeC[shape_y_]; out->y_waves = kShapeW[shape_y_];
+    out->explicit_us = tuned_us_[0]; out->x_lines_us = tuned_us_[1]; out->y_lines_us = tuned_us_[2];
+}
+
+void ShallowWaterEngine::setTuning(const cel_tuning &t)
+{
+    auto shape_of = [](int c, int w) {
+        for (int i = 0; i < kNumShapes; ++i)
+            if (kShapeC[i] == c && kShapeW[i] == w) return i;
+        return -1;
+    };
+    if (t.band_rows != 4 && t.band_rows != 8 && t.band_rows != 16) throw Error(CEL_EINVAL, "band_rows must be 4, 8 or 16");
+    const int sx = shape_of(t.x_chunk, t.x_waves), sy = shape_of(t.y_chunk, t.y_waves);
+    if (sx < 0 || sy < 0) throw Error(CEL_EINVAL, "unsupported line-solve block shape");
+    if (shape_capacity(sx) < nx_ || shape_capacity(sy) < ny_)
+        throw Error(CEL_EINVAL, "line-solve block shape too small for the line length");
+    sync();
+    clearGraphs();
+    rows_ = t.band_rows;
+    shape_x_ = sx;
+    shape_y_ = sy;
+    tuned_us_[0] = tuned_us_[1] = tuned_us_[2] = 0.f;
+}
+
+// Times each candidate on the current state.  The explicit kernel writes the next state's
+// w, the line right-hand sides and the scratch history slot; the solves write the next
+// state's hu and hv: all of it is rewritten by the next step, so tuning changes no result.
+void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
+{
+    if (repeats < 1 || repeats > 1000) throw Error(CEL_EINVAL, "repeats must lie in [1, 1000]");
+    sync();
+    clearGraphs();
+    const int scheme = s_.time_scheme;     // the steady-state scheme
+    auto time_it = [&](auto &&launch) {
+        launch();                           // warm-up
+        check(hipEventRecord(ev_[6], stream_), "event");
+        for (int r = 0; r < repeats; ++r) launch();
+        check(hipEventRecord(ev_[7], stream_), "event");
+        check(hipEventSynchronize(ev_[7]), "event sync");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, ev_[6], ev_[7]), "event");
+        return 1000.f * ms / repeats;
+    };
+    const int keep_rows = rows_;
+    float best = 0.f;
+    int best_rows = rows_;
+    for (int rows : {4, 8, 16}) {
+        rows_ = rows;
+        ExplicitParams p;
+        fillExplicit(p);
+        const float us = time_it([&] { check(launch_explicit(scheme, comp_, rows, p, stream_), "explicit_step"); });
+        if (rows == 4 || us < best) { best = us; best_rows = rows; }
+    }
+    rows_ = best_rows;
+    tuned_us_[0] = best;
+    (void)keep_rows;
+    LineParams lp;
+    fillLines(lp);
+    IrregularAll none;
+    std::memset(&none, 0, sizeof(none));
+    for (int dir = 0; dir < 2; ++dir) {
+        const int n = dir == 0 ? nx_ : ny_;
+        LineParams q = lp;
+        if (dir == 0) q.nlines_y = 0; else q.nlines_x = 0;
+        int best_shape = dir == 0 ? shape_x_ : shape_y_;
+        float best_us = -1.f;
+        for (int sh = 0; sh < kNumShapes; ++sh) {
+            const int cap = shape_capacity(sh);
+            if (cap < n || cap > std::max(4 * n, 64)) continue;
+            const int sx = dir == 0 ? sh : shape_x_, sy = dir == 0 ? shape_y_ : sh;
+            const float us = time_it([&] { check(launch_solve(comp_, sx, sy, q, none, stream_), "solve_lines"); });
+            if (best_us < 0.f || us < best_us) { best_us = us; best_shape = sh; }
+        }
+        if (dir == 0) shape_x_ = best_shape; else shape_y_ = best_shape;
+        tuned_us_[1 + dir] = best_us;
+    }
+    if (out) getTuning(out);
 }
 
 void ShallowWaterEngine::coefficients(float *ax, float *ay)

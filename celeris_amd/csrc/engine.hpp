@@ -24,8 +24,11 @@ struct Error : std::runtime_error {
     Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
 };
 
-hipError_t launch_explicit(int scheme, bool comp, const ExplicitParams &p, hipStream_t st);
-hipError_t launch_solve(bool comp, const LineParams &p, const IrregularAll &irr, hipStream_t st);
+hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st);
+hipError_t launch_solve(bool comp, int shape_x, int shape_y, const LineParams &p, const IrregularAll &irr,
+                        hipStream_t st);
+int default_shape(int n);
+int shape_capacity(int shape);
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
 hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st);
@@ -52,6 +55,7 @@ struct SideState {
 struct StepLaunch {
     struct Copy { float *dst; const float *src; size_t bytes; };
     int scheme;
+    int rows, shape_x, shape_y;     // explicit band rows, line-solve shapes
     int ncopy;
     Copy copy[8];
     ExplicitParams ep;
@@ -91,9 +95,13 @@ public:
     void stats(cel_stats *out);
     void resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out);
     void setGraphSteps(int k);                                // steps per captured block (0: off)
-    // plans the next step (kernel parameters from the current host state) and advances
-    // the host state as the step will
-    void planStep(StepLaunch &L);
+    // plans the next step (kernel parameters from the current host state) and, if advance,
+    // moves the host state on as the step will
+    void planStep(StepLaunch &L, bool advance = true);
+    // tile shape and line-solve shapes: timed choice, current choice, caller's choice
+    void autotune(int repeats, cel_tuning *out);
+    void getTuning(cel_tuning *out) const;
+    void setTuning(const cel_tuning &t);
 
 private:
     void allocate();
@@ -157,6 +165,10 @@ private:
     float target_cfl_ = 0.f;
     void runStats(double *out9);
     void deriveStats(const double *raw, cel_stats *out) const;
+
+    // explicit band rows and line-solve shapes (indices into kShapeC / kShapeW)
+    int rows_ = kDefaultBandRows, shape_x_ = 0, shape_y_ = 0;
+    float tuned_us_[3] = {};
 
     // scheme state (engine.cpp:3254-3269, 1476-1519)
     int cur_ = 0;

@@ -211,19 +211,18 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
 
 // LDS of one explicit tile: the band's transposed hv (and the corrector's start hv),
 // and the y-line right-hand sides, stored transposed after the band.
-template <bool CORRECTOR>
+template <bool CORRECTOR, int R>
 struct ExplicitSmem {
-    float hv[kBandRows + 4][64];
-    float chv[CORRECTOR ? kBandRows + 4 : 1][64];
-    float dy[kBandRows][64];
+    float hv[R + 4][64];
+    float chv[CORRECTOR ? R + 4 : 1][64];
+    float dy[R][64];
 };
 
 // One wave computes interior columns [x0, x0 + 60) of rows [ys, ys + R) of one scenario.
 // SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
-template <int SCHEME, bool COMP, bool CS>
-__device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS> &sm)
+template <int SCHEME, bool COMP, int R, bool CS>
+__device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS, R> &sm)
 {
-    constexpr int R = kBandRows;
     float (*hvblk)[64] = sm.hv;
     float (*chvblk)[64] = sm.chv;
     float (*dystg)[64] = sm.dy;
@@ -522,17 +521,17 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
 }
 
-template <int SCHEME, bool COMP>
+template <int SCHEME, bool COMP, int R>
 __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 {
-    __shared__ ExplicitSmem<SCHEME == 2> sm;
+    __shared__ ExplicitSmem<SCHEME == 2, R> sm;
     // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of
     // tiles (strip-fastest) so neighbouring strips and bands share that XCD's L2
     const int ntiles = p.nstrips * p.nbands;
     const int per_xcd = (ntiles + 7) / 8;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
-    explicit_tile<SCHEME, COMP>(p, tile, sm);
+    explicit_tile<SCHEME, COMP, R>(p, tile, sm);
 }
 
 // ------------------------------------------------------------------ line solves
@@ -968,10 +967,11 @@ __device__ __forceinline__ int find_job(const int *starts, int njobs, int v)
     return lo;
 }
 
+template <int R>
 __global__ __launch_bounds__(64) void explicit_batched(const ExplicitJob *__restrict__ jobs,
                                                       const int *__restrict__ starts, int njobs)
 {
-    __shared__ ExplicitSmem<false> sm;
+    __shared__ ExplicitSmem<false, R> sm;
     const int total = starts[njobs];
     const int per_xcd = (total + 7) / 8;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -980,12 +980,12 @@ __global__ __launch_bounds__(64) void explicit_batched(const ExplicitJob *__rest
     const ExplicitJob &J = jobs[j];
     const int tile = t - starts[j];
     switch (J.scheme * 2 + J.comp) {
-    case 0: explicit_tile<0, false>(J.p, tile, sm); break;
-    case 1: explicit_tile<0, true>(J.p, tile, sm); break;
-    case 2: explicit_tile<1, false>(J.p, tile, sm); break;
-    case 3: explicit_tile<1, true>(J.p, tile, sm); break;
-    case 6: explicit_tile<3, false>(J.p, tile, sm); break;
-    case 7: explicit_tile<3, true>(J.p, tile, sm); break;
+    case 0: explicit_tile<0, false, R>(J.p, tile, sm); break;
+    case 1: explicit_tile<0, true, R>(J.p, tile, sm); break;
+    case 2: explicit_tile<1, false, R>(J.p, tile, sm); break;
+    case 3: explicit_tile<1, true, R>(J.p, tile, sm); break;
+    case 6: explicit_tile<3, false, R>(J.p, tile, sm); break;
+    case 7: explicit_tile<3, true, R>(J.p, tile, sm); break;
     default: break;   // corrector steps are not batched (rejected on the host)
     }
 }
@@ -1202,92 +1202,119 @@ __global__ void export_coefficients(const StaticParams p, float *ax, float *ay)
 
 namespace cel {
 
-hipError_t launch_explicit(int scheme, bool comp, const ExplicitParams &p, hipStream_t st)
+hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st)
 {
     const dim3 grid(8 * ((p.nstrips * p.nbands + 7) / 8)), block(64);
-#define CEL_EXPLICIT(S, CP) hipLaunchKernelGGL((explicit_step<S, CP>), grid, block, 0, st, p)
+#define CEL_EXPLICIT(S, CP, R) hipLaunchKernelGGL((explicit_step<S, CP, R>), grid, block, 0, st, p)
+#define CEL_EXPLICIT_R(S, CP)                                  \
+    do {                                                       \
+        if (rows == 4) CEL_EXPLICIT(S, CP, 4);                 \
+        else if (rows == 16) CEL_EXPLICIT(S, CP, 16);          \
+        else CEL_EXPLICIT(S, CP, 8);                           \
+    } while (0)
+    if (rows != 4 && rows != 8 && rows != 16) return hipErrorInvalidValue;
     if (comp) {
         switch (scheme) {
-        case 0: CEL_EXPLICIT(0, true); break;
-        case 1: CEL_EXPLICIT(1, true); break;
-        case 3: CEL_EXPLICIT(3, true); break;
+        case 0: CEL_EXPLICIT_R(0, true); break;
+        case 1: CEL_EXPLICIT_R(1, true); break;
+        case 3: CEL_EXPLICIT_R(3, true); break;
         default: return hipErrorInvalidValue;   // no compensated corrector
         }
     } else {
         switch (scheme) {
-        case 0: CEL_EXPLICIT(0, false); break;
-        case 1: CEL_EXPLICIT(1, false); break;
-        case 2: CEL_EXPLICIT(2, false); break;
-        default: CEL_EXPLICIT(3, false); break;
+        case 0: CEL_EXPLICIT_R(0, false); break;
+        case 1: CEL_EXPLICIT_R(1, false); break;
+        case 2: CEL_EXPLICIT_R(2, false); break;
+        default: CEL_EXPLICIT_R(3, false); break;
         }
     }
+#undef CEL_EXPLICIT_R
 #undef CEL_EXPLICIT
     return hipGetLastError();
 }
 
-hipError_t launch_solve(bool comp, const LineParams &p, const IrregularAll &irr, hipStream_t st)
+// Default line-solve shape for a line of n unknowns: (1,1) (2,1) (4,1) (8,1) (8,2) (8,4)
+// (16,4) (16,8) by length class.
+int default_shape(int n)
 {
-    const int nmax = p.nx > p.ny ? p.nx : p.ny;
-    if (nmax <= 64 * 8 * 16) {
-        int irr_lines = 0;
-        for (int s = 0; s < 4; ++s) irr_lines += irr.count[s];
-        // lines of up to 64 W C unknowns: C per thread in registers, W waves per line
-#define CEL_SOLVE(C, W)                                                                              \
-    do {                                                                                            \
-        const dim3 grid(p.nlines_x + p.nlines_y + (irr_lines + (W) - 1) / (W)), block(64 * (W));    \
-        if (comp) hipLaunchKernelGGL((solve_lines_block<C, W, true>), grid, block, 0, st, p, irr);   \
-        else hipLaunchKernelGGL((solve_lines_block<C, W, false>), grid, block, 0, st, p, irr);       \
-    } while (0)
-        if (nmax <= 64) CEL_SOLVE(1, 1);
-        else if (nmax <= 128) CEL_SOLVE(2, 1);
-        else if (nmax <= 256) CEL_SOLVE(4, 1);
-        else if (nmax <= 512) CEL_SOLVE(8, 1);
-        else if (nmax <= 1024) CEL_SOLVE(8, 2);
-        else if (nmax <= 2048) CEL_SOLVE(8, 4);
-        else if (nmax <= 4096) CEL_SOLVE(16, 4);
-        else CEL_SOLVE(16, 8);
-#undef CEL_SOLVE
-        return hipGetLastError();
-    }
-    return hipErrorInvalidValue;   // lines longer than CEL_MAX_LINE are rejected at creation
+    return n <= 64 ? 0 : n <= 128 ? 1 : n <= 256 ? 3 : n <= 512 ? 5 : n <= 1024 ? 9 : n <= 2048 ? 12 : n <= 4096 ? 14
+         : 16;
 }
 
-hipError_t launch_explicit_batched(const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
-                                   hipStream_t st)
+int shape_capacity(int shape) { return 64 * kShapeC[shape] * kShapeW[shape]; }
+
+// Dispatch on a line-solve shape index (kShapeC/kShapeW).
+#define CEL_SHAPE_SWITCH(shape, MACRO)                                                              \
+    switch (shape) {                                                                                \
+    case 0: MACRO(1, 1); break;   case 1: MACRO(2, 1); break;   case 2: MACRO(1, 2); break;          \
+    case 3: MACRO(4, 1); break;   case 4: MACRO(2, 2); break;   case 5: MACRO(8, 1); break;          \
+    case 6: MACRO(4, 2); break;   case 7: MACRO(2, 4); break;   case 8: MACRO(16, 1); break;         \
+    case 9: MACRO(8, 2); break;   case 10: MACRO(4, 4); break;  case 11: MACRO(16, 2); break;        \
+    case 12: MACRO(8, 4); break;  case 13: MACRO(4, 8); break;  case 14: MACRO(16, 4); break;        \
+    case 15: MACRO(8, 8); break;  default: MACRO(16, 8); break;                                      \
+    }
+
+static void launch_solve_shape(int shape, bool comp, const LineParams &lp, const IrregularAll &ir, int nirr,
+                               hipStream_t st)
 {
-    if (total_tiles > 0)
-        hipLaunchKernelGGL(explicit_batched, dim3(8 * ((total_tiles + 7) / 8)), dim3(64), 0, st, jobs, starts, njobs);
+    const int W = kShapeW[shape];
+    const dim3 grid(lp.nlines_x + lp.nlines_y + (nirr + W - 1) / W);
+#define CEL_SOLVE(C, W_)                                                                                    \
+    do {                                                                                                    \
+        if (comp) hipLaunchKernelGGL((solve_lines_block<C, W_, true>), grid, dim3(64 * W_), 0, st, lp, ir); \
+        else hipLaunchKernelGGL((solve_lines_block<C, W_, false>), grid, dim3(64 * W_), 0, st, lp, ir);     \
+    } while (0)
+    CEL_SHAPE_SWITCH(shape, CEL_SOLVE)
+#undef CEL_SOLVE
+}
+
+// One launch per line direction (x-lines with the irregular-wave lines, then y-lines), each
+// with its own block shape; one launch when both directions share a shape.
+hipError_t launch_solve(bool comp, int shape_x, int shape_y, const LineParams &p, const IrregularAll &irr,
+                        hipStream_t st)
+{
+    if (shape_x < 0 || shape_x >= kNumShapes || shape_y < 0 || shape_y >= kNumShapes) return hipErrorInvalidValue;
+    if (shape_capacity(shape_x) < p.nx || shape_capacity(shape_y) < p.ny) return hipErrorInvalidValue;
+    int irr_lines = 0;
+    for (int s = 0; s < 4; ++s) irr_lines += irr.count[s];
+    if (shape_x == shape_y && p.nlines_x > 0 && p.nlines_y > 0) {
+        launch_solve_shape(shape_x, comp, p, irr, irr_lines, st);
+    } else {
+        LineParams px = p, py = p;
+        px.nlines_y = 0;
+        py.nlines_x = 0;
+        IrregularAll none = irr;
+        for (int s = 0; s < 4; ++s) none.count[s] = 0;
+        if (px.nlines_x > 0 || irr_lines > 0) launch_solve_shape(shape_x, comp, px, irr, irr_lines, st);
+        if (py.nlines_y > 0) launch_solve_shape(shape_y, comp, py, none, 0, st);
+    }
     return hipGetLastError();
 }
 
-int solve_class(int nmax)
+hipError_t launch_explicit_batched(int rows, const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
+                                   hipStream_t st)
 {
-    return nmax <= 64 ? 0 : nmax <= 128 ? 1 : nmax <= 256 ? 2 : nmax <= 512 ? 3 : nmax <= 1024 ? 4 : nmax <= 2048 ? 5
-         : nmax <= 4096 ? 6 : 7;
+    if (total_tiles <= 0) return hipSuccess;
+    const dim3 grid(8 * ((total_tiles + 7) / 8));
+    if (rows == 4) hipLaunchKernelGGL(explicit_batched<4>, grid, dim3(64), 0, st, jobs, starts, njobs);
+    else if (rows == 16) hipLaunchKernelGGL(explicit_batched<16>, grid, dim3(64), 0, st, jobs, starts, njobs);
+    else if (rows == 8) hipLaunchKernelGGL(explicit_batched<8>, grid, dim3(64), 0, st, jobs, starts, njobs);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
-// threads of one line per class: 64 W
-int solve_class_waves(int cls) { static const int w[8] = {1, 1, 1, 1, 2, 4, 4, 8}; return w[cls]; }
-
-hipError_t launch_solve_batched(int cls, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
+hipError_t launch_solve_batched(int shape, const SolveJob *jobs, const int *starts, int njobs, int total_blocks,
                                 hipStream_t st)
 {
     if (total_blocks <= 0) return hipSuccess;
+    if (shape < 0 || shape >= kNumShapes) return hipErrorInvalidValue;
     const dim3 grid(total_blocks);
 #define CEL_SB(C, W) hipLaunchKernelGGL((solve_batched<C, W>), grid, dim3(64 * (W)), 0, st, jobs, starts, njobs)
-    switch (cls) {
-    case 0: CEL_SB(1, 1); break;
-    case 1: CEL_SB(2, 1); break;
-    case 2: CEL_SB(4, 1); break;
-    case 3: CEL_SB(8, 1); break;
-    case 4: CEL_SB(8, 2); break;
-    case 5: CEL_SB(8, 4); break;
-    case 6: CEL_SB(16, 4); break;
-    default: CEL_SB(16, 8); break;
-    }
+    CEL_SHAPE_SWITCH(shape, CEL_SB)
 #undef CEL_SB
     return hipGetLastError();
 }
+#undef CEL_SHAPE_SWITCH
 
 hipError_t launch_boundary_batched(const BoundaryJob *jobs, const int *starts, int njobs, int total_cells,
                                    int mirror_pass, hipStream_t st)

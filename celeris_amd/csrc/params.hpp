@@ -10,11 +10,15 @@
 
 namespace cel {
 
-#ifndef CEL_BAND_ROWS
-#define CEL_BAND_ROWS 8
-#endif
-constexpr int kBandRows = CEL_BAND_ROWS;   // rows one wave marches through in the explicit kernel
+// Rows one wave marches through in the explicit kernel: 4, 8 or 16 (chosen per engine by
+// the autotuner; 8 by default).
+constexpr int kDefaultBandRows = 8;
 constexpr int kStripCols = 60;  // interior columns one wave produces (64 lanes - 2 halo each side)
+
+// Line-solve block shapes: C unknowns per thread, W waves per line (64 W C >= line length).
+constexpr int kNumShapes = 17;
+constexpr int kShapeC[kNumShapes] = {1, 2, 1, 4, 2, 8, 4, 2, 16, 8, 4, 16, 8, 4, 16, 8, 16};
+constexpr int kShapeW[kNumShapes] = {1, 1, 2, 1, 2, 1, 2, 4, 1, 2, 4, 2, 4, 8, 4, 8, 8};
 
 struct ExplicitParams {
     int nx, ny, P, PT;

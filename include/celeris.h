@@ -229,6 +229,10 @@ int cel_batch_remove(cel_batch *b, int id);
 int cel_batch_step(cel_batch *b, int nsteps);
 /* Steps per captured block (0: direct launches). */
 int cel_batch_set_graph_steps(cel_batch *b, int steps);
+/* Times the batched explicit launch with band heights 4, 8 and 16 over the current
+ * scenarios and keeps the fastest (*band_rows); with tune_lines, autotunes each
+ * scenario's line-solve shapes as cel_engine_autotune does.  Results are unchanged. */
+int cel_batch_autotune(cel_batch *b, int repeats, int tune_lines, int *band_rows);
 int cel_batch_sync(cel_batch *b);
 /* Runs nsteps and returns the elapsed device time (HIP events on the batch stream). */
 int cel_batch_timed_steps(cel_batch *b, int nsteps, float *ms);
@@ -239,6 +243,25 @@ int cel_batch_get_state(cel_batch *b, int id, float *state3);
 int cel_batch_get_state_f64(cel_batch *b, int id, double *state3);
 int cel_batch_stats(cel_batch *b, int id, cel_stats *out);
 int cel_batch_time(cel_batch *b, int id, double *t);
+
+/* Work decomposition of an engine's kernels.  Results do not depend on it: every cell and
+ * every elimination of the line solves is computed the same way whatever the choice.  No
+ * counterpart in the reference. */
+typedef struct {
+    int band_rows;              /* rows per wave in the explicit stencil: 4, 8 or 16 */
+    int x_chunk, x_waves;       /* x-line solve: unknowns per thread (1..16), waves per line (1..8) */
+    int y_chunk, y_waves;       /* y-line solve */
+    float explicit_us, x_lines_us, y_lines_us;   /* timings behind an autotuned choice (else 0) */
+} cel_tuning;
+
+/* Times every band height and every line-solve block shape that fits the engine's line
+ * lengths on the current state (each candidate launched `repeats` times; the kernels
+ * only write buffers the next step overwrites, so the state is unchanged), keeps the
+ * fastest and reports it. */
+int cel_engine_autotune(cel_engine *e, int repeats, cel_tuning *out);
+int cel_engine_get_tuning(cel_engine *e, cel_tuning *out);
+/* Applies a tuning (e.g. one cached from an earlier run on the same grid). */
+int cel_engine_set_tuning(cel_engine *e, const cel_tuning *t);
 
 /* Level-0 tridiagonal coefficients as the engine uses them (engine.cpp:2638-2694,
  * 2781-2826): three planes A, B, C for x and for y, computed on the device. */

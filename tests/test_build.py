@@ -34,5 +34,9 @@ def test_no_kernel_uses_scratch(resource_report):
 
 
 def test_explicit_step_occupancy(resource_report):
-    occ = [v["Occupancy"] for k, v in resource_report.items() if "explicit_step" in k]
-    assert occ and min(occ) >= 4, occ
+    # 4 waves per SIMD at the default band height; the corrector's 16-row variant holds three
+    # LDS planes of 20 rows and is LDS-limited to 3
+    occ = {k: v["Occupancy"] for k, v in resource_report.items() if "explicit" in k}
+    default = [o for k, o in occ.items() if "ELi8E" in k]
+    assert default and min(default) >= 4, occ
+    assert min(occ.values()) >= 3, occ

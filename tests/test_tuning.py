@@ -1,0 +1,72 @@
+"""Work decomposition (band height of the explicit stencil, line-solve block shape) and
+the autotuner.  Every choice computes each cell and every elimination the same way, so
+states must be bit-identical whatever the tuning: fixed choices, autotuned engines and
+autotuned batches against default engines."""
+import numpy as np
+import pytest
+
+from celeris_amd import (IRREGULARWAVES, SINEWAVE, SOLID, SPONGE, STATE_COMPENSATED, Batch, Boundary, CeleriError,
+                         ShallowWaterEngine)
+from celeris_amd import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def case(nx, ny, seed, comp=False):
+    rng = np.random.default_rng(seed)
+    waves = np.stack([rng.uniform(0.005, 0.02, 4), rng.uniform(1.0, 3.0, 4), rng.uniform(-0.4, 0.4, 4),
+                      rng.uniform(0, 2 * np.pi, 4)], axis=1).astype(np.float32)
+    bnd = [Boundary(IRREGULARWAVES, 2, waves=waves), Boundary(SPONGE, 9), Boundary(SOLID, 2),
+           Boundary(SINEWAVE, 2, amplitude=0.02, period=1.3, theta=0.2)]
+    sc = S.small(nx=nx, ny=ny, seed=seed, bnd=bnd)
+    if comp:
+        sc.s.state_precision = STATE_COMPENSATED
+    return sc
+
+
+def run(sc, steps, tuning=None, autotune=False):
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    if tuning:
+        e.set_tuning(*tuning)
+    e.timestep(steps // 2)
+    if autotune:
+        t = e.autotune(repeats=3)
+        assert t["band_rows"] in (4, 8, 16) and t["explicit_us"] > 0
+    e.timestep(steps - steps // 2)
+    st = e.get_state_f64()
+    e.close()
+    return st
+
+
+@pytest.mark.parametrize("tuning", [(4, 1, 2, 2, 2), (16, 2, 1, 1, 2), (8, 1, 2, 4, 1), (16, 4, 4, 8, 2)])
+def test_fixed_tunings_bit_identical(tuning):
+    sc = case(100, 70, 41)
+    assert np.array_equal(run(sc, 60), run(sc, 60, tuning=tuning))
+
+
+@pytest.mark.parametrize("nx,ny,comp", [(180, 90, False), (600, 40, True), (30, 300, False)])
+def test_autotuned_engine_bit_identical(nx, ny, comp):
+    sc = case(nx, ny, 42, comp)
+    assert np.array_equal(run(sc, 50), run(sc, 50, autotune=True))
+
+
+def test_invalid_tunings_rejected():
+    sc = case(300, 40, 43)
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    for bad in [(5, 8, 1, 1, 1), (8, 1, 1, 1, 1), (8, 3, 2, 1, 1), (8, 16, 16, 1, 1)]:
+        with pytest.raises(CeleriError):
+            e.set_tuning(*bad)
+    e.close()
+
+
+def test_autotuned_batch_bit_identical():
+    scs = [case(70, 50, 44), case(300, 120, 45, comp=True), case(40, 260, 46)]
+    b = Batch()
+    ids = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs]
+    b.step(20)
+    rows = b.autotune(repeats=3, tune_lines=True)
+    assert rows in (4, 8, 16)
+    b.step(30)
+    for sid, sc in zip(ids, scs):
+        assert np.array_equal(b.get_state_f64(sid), run(sc, 50))
+    b.close()
