@@ -96,9 +96,9 @@ def ensemble_line(args):
     cells = sum(m.s.nx * m.s.ny for m in members)
     b = Batch(graph_steps=args.graph_steps)
     ids = [b.add(m.s, m.boundaries, m.bottom, m.state) for m in members]
-    b.step(min(args.warmup, 10))
+    b.step(args.warmup)
     band_rows = b.autotune(args.autotune, tune_lines=True) if args.autotune > 0 else None
-    b.step(args.warmup - min(args.warmup, 10))
+    b.step(10)
     b.sync()
     t0 = time.perf_counter()
     ms = b.timed_steps(args.steps)
@@ -188,9 +188,9 @@ def main():
     eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     cells = sc.s.nx * sc.s.ny
     eng.set_graph_steps(args.graph_steps)
-    eng.timestep(min(args.warmup, 10))
+    eng.timestep(args.warmup)
     tuning = eng.autotune(args.autotune) if args.autotune > 0 else eng.tuning()
-    eng.timestep(args.warmup - min(args.warmup, 10))
+    eng.timestep(10)
     eng.sync()
 
     # timed region: K steps bracketed by device synchronisation, wall clock and HIP events

@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("CELERIS_AMD_LIB", os.path.join(HERE, "libceleris_amd.
 SOLID, SPONGE, SINEWAVE, IRREGULARWAVES, UNIFORMTIMESERIES = range(5)
 WEST, EAST, SOUTH, NORTH = range(4)
 EULER, PREDICTOR, CORRECTOR, PREDICTOR_ADAPTIVE = range(4)
-STATE_FP32, STATE_COMPENSATED = range(2)
+STATE_FP32, STATE_COMPENSATED, STATE_FP64 = range(3)
 STATUS = {0: "CEL_OK", 1: "CEL_EINVAL", 2: "CEL_ENOMEM", 3: "CEL_EDEVICE", 4: "CEL_ENUMERIC", 5: "CEL_ESTATE"}
 
 _F = ctypes.POINTER(ctypes.c_float)

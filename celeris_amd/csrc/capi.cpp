@@ -6,6 +6,9 @@
 
 #include "batch.hpp"
 #include "engine.hpp"
+#define CEL_FP64
+#include "engine.hpp"
+#undef CEL_FP64
 
 namespace cel {
 void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
@@ -13,7 +16,7 @@ void initial_state(const cel_settings &s, const float *bottom3, const cel_solito
 }
 
 struct cel_engine {
-    cel::ShallowWaterEngine *impl;
+    cel::EngineApi *impl;
 };
 
 struct cel_batch {
@@ -45,7 +48,7 @@ int guarded(F &&f)
     }
 }
 
-cel::ShallowWaterEngine &eng(cel_engine *e)
+cel::EngineApi &eng(cel_engine *e)
 {
     if (!e || !e->impl) throw cel::Error(CEL_EINVAL, "null engine");
     return *e->impl;
@@ -89,7 +92,8 @@ int cel_engine_create(const cel_settings *s, const cel_boundary bnd[4], const fl
         *out = nullptr;
         auto *e = new cel_engine{nullptr};
         try {
-            e->impl = new cel::ShallowWaterEngine(*s, bnd, bottom3, state3);
+            if (s->state_precision == CEL_STATE_FP64) e->impl = new cel64::ShallowWaterEngine(*s, bnd, bottom3, state3);
+            else e->impl = new cel::ShallowWaterEngine(*s, bnd, bottom3, state3);
         } catch (...) {
             delete e;
             throw;
@@ -126,7 +130,7 @@ int cel_engine_time(cel_engine *e, double *t)
 {
     return guarded([&] {
         if (!t) throw cel::Error(CEL_EINVAL, "null output");
-        *t = eng(e).time();
+        *t = eng(e).timeValue();
     });
 }
 
@@ -176,7 +180,8 @@ int cel_estimate_device_bytes(const cel_settings *s, const cel_boundary bnd[4], 
 {
     return guarded([&] {
         if (!s || !bnd || !bytes) throw cel::Error(CEL_EINVAL, "null argument");
-        *bytes = cel::ShallowWaterEngine::estimateBytes(*s, bnd);
+        *bytes = s->state_precision == CEL_STATE_FP64 ? cel64::ShallowWaterEngine::estimateBytes(*s, bnd)
+                                                      : cel::ShallowWaterEngine::estimateBytes(*s, bnd);
     });
 }
 

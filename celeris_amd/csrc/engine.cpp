@@ -6,26 +6,27 @@
 //                           engine.cpp:1060-1233 and 1811-2151)
 //   boundary_apply x2      (four sides in the order E, N, W, S, engine.cpp:1235-1438)
 // then advances the scheme state machine exactly as engine.cpp:1476-1521.
+// Compiled once per working precision (real = float in namespace cel, double in cel64).
 #include "engine.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
-namespace cel {
+namespace CEL_NS {
 
 namespace {
 
-constexpr float kMinDepth = 0.0001f;   // MIN_DEPTH, compute.hlsl:1016
+constexpr real kMinDepth = real(0.0001);   // MIN_DEPTH, compute.hlsl:1016
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-float sponge_function(float PI, float L, float x)   // compute.hlsl:1011-1014
+real sponge_function(real PI, real L, real x)   // compute.hlsl:1011-1014
 {
-    return 0.5f + 0.5f * std::cos(PI * std::max(L - x, 0.f) / L);
+    return real(0.5) + real(0.5) * std::cos(PI * std::max(L - x, real(0)) / L);
 }
 
-float wavenumber(float omega, float d, float g)     // compute.hlsl:1080-1084
+real wavenumber(real omega, real d, real g)     // compute.hlsl:1080-1084
 {
     return omega * omega / (g * std::sqrt(std::tanh(omega * omega * d / g)));
 }
@@ -58,9 +59,13 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
     if (s.time_scheme != CEL_PREDICTOR && s.time_scheme != CEL_PREDICTOR_ADAPTIVE)
         throw Error(CEL_EINVAL, "time_scheme must be CEL_PREDICTOR or CEL_PREDICTOR_ADAPTIVE");
     if (s.correction_steps_num < 0) throw Error(CEL_EINVAL, "correction_steps_num must be >= 0");
+#ifdef CEL_FP64
+    if (s.state_precision != CEL_STATE_FP64) throw Error(CEL_EINVAL, "an FP64 engine needs state_precision CEL_STATE_FP64");
+#else
     if (s.state_precision != CEL_STATE_FP32 && s.state_precision != CEL_STATE_COMPENSATED)
-        throw Error(CEL_EINVAL, "state_precision must be CEL_STATE_FP32 or CEL_STATE_COMPENSATED");
+        throw Error(CEL_EINVAL, "state_precision must be CEL_STATE_FP32, CEL_STATE_COMPENSATED or CEL_STATE_FP64");
     comp_ = s.state_precision == CEL_STATE_COMPENSATED;
+#endif
     if (comp_ && s.correction_steps_num > 0)
         throw Error(CEL_EINVAL, "the compensated state update does not take corrector steps");
     nx_ = s.nx; ny_ = s.ny; X_ = nx_ + 4; Y_ = ny_ + 4;
@@ -70,9 +75,9 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
     Ly_ = count_of_matrices(ny_);
     shape_x_ = default_shape(nx_);
     shape_y_ = default_shape(ny_);
-    dx_ = s.width / (float)(nx_ - 1);
-    dy_ = s.length / (float)(ny_ - 1);
-    PI_ = 4.0f * std::atan(1.0f);
+    dx_ = (real)s.width / (real)(nx_ - 1);
+    dy_ = (real)s.length / (real)(ny_ - 1);
+    PI_ = real(4) * std::atan(real(1));
     for (int i = 0; i < 4; ++i) {
         if (bnd[i].type < CEL_SOLID || bnd[i].type > CEL_UNIFORMTIMESERIES) throw Error(CEL_EINVAL, "bad boundary type");
         side_[i].cfg = bnd[i];
@@ -106,8 +111,8 @@ ShallowWaterEngine::~ShallowWaterEngine()
 
 void ShallowWaterEngine::allocate()
 {
-    const size_t plane = (size_t)P_ * Y_ * sizeof(float);
-    const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
+    const size_t plane = (size_t)P_ * Y_ * sizeof(real);
+    const size_t tplane = (size_t)PT_ * X_ * sizeof(real);
     // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; static; compensated w_lo, hu_lo x2
     const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6 + (comp_ ? 4 : 0);
     const int ncol = 2 + 1 + 1 + 2 + (comp_ ? 2 : 0);         // hvT x2; DyT; ChvT; KyT, QyT; hvT_lo x2
@@ -117,8 +122,8 @@ void ShallowWaterEngine::allocate()
     arena_ = static_cast<char *>(mem);
     check(hipMemsetAsync(arena_, 0, arena_bytes_, stream_), "hipMemset(state)");
     char *q = arena_;
-    auto rp = [&]() { float *r = reinterpret_cast<float *>(q); q += plane; return r; };
-    auto cp = [&]() { float *r = reinterpret_cast<float *>(q); q += tplane; return r; };
+    auto rp = [&]() { real *r = reinterpret_cast<real *>(q); q += plane; return r; };
+    auto cp = [&]() { real *r = reinterpret_cast<real *>(q); q += tplane; return r; };
     w_[0] = rp(); w_[1] = rp(); hu_[0] = rp(); hu_[1] = rp();
     BY_ = rp(); BX_ = rp(); BA_ = rp(); Dx_ = rp();
     for (auto &g : G_) for (auto &c : g) c = rp();
@@ -147,7 +152,7 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
     const int X = s.nx + 4, Y = s.ny + 4;
     const size_t P = round_up((size_t)X + 16, 64), PT = round_up((size_t)Y + 16, 64);
     const bool comp = s.state_precision == CEL_STATE_COMPENSATED;
-    const size_t plane = P * Y * sizeof(float), tplane = PT * X * sizeof(float);
+    const size_t plane = P * Y * sizeof(real), tplane = PT * X * sizeof(real);
     size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 6 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 2 + (comp ? 2 : 0));
     const int tiles = ((4 * (s.nx / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (s.ny / 4) + kStatsTileY - 1) / kStatsTileY);
     b += ((size_t)tiles + 1) * kStatsValues * sizeof(double);
@@ -155,13 +160,14 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
         const cel_boundary &c = bnd[side];
         const size_t ncell = (size_t)c.width * ((side == CEL_WEST || side == CEL_EAST) ? Y : X);
         if (ncell == 0) continue;
-        if (c.type == CEL_SPONGE) b += ncell * 4;
-        else if (c.type == CEL_SINEWAVE) b += ncell * 5 * 4;
+        const size_t e = sizeof(real);
+        if (c.type == CEL_SPONGE) b += ncell * e;
+        else if (c.type == CEL_SINEWAVE) b += ncell * 5 * e;
         else if (c.type == CEL_IRREGULARWAVES) {
             const int L = count_of_matrices((side == CEL_WEST || side == CEL_EAST) ? s.nx : s.ny);
             const int nact = std::min(c.n_waves, 1 << (L - 1));
             const size_t nw1 = (size_t)std::max(nact, 1);
-            b += ncell * nw1 * 16 + nw1 * 12 + ncell + ncell * 12;
+            b += ncell * nw1 * 4 * e + nw1 * 3 * e + ncell + ncell * 3 * e;
         }
     }
     return b;
@@ -170,11 +176,13 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
 void ShallowWaterEngine::uploadBottom(const float *bottom3)
 {
     const size_t N = (size_t)X_ * Y_;
-    float *dst[3] = {BY_, BX_, BA_};
+    const size_t e = sizeof(real);
+    const std::vector<real> b(bottom3, bottom3 + 3 * N);
+    real *dst[3] = {BY_, BX_, BA_};
     for (int c = 0; c < 3; ++c)
-        check(hipMemcpy2DAsync(dst[c], (size_t)P_ * 4, bottom3 + c * N, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+        check(hipMemcpy2DAsync(dst[c], (size_t)P_ * e, b.data() + c * N, (size_t)X_ * e, (size_t)X_ * e, Y_,
                                hipMemcpyHostToDevice, stream_), "upload bottom");
-    hostBA_.assign(bottom3 + 2 * N, bottom3 + 3 * N);
+    hostBA_.assign(b.begin() + 2 * N, b.begin() + 3 * N);
     StaticParams sp;
     fillStatic(sp);
     check(launch_static_fields(sp, stream_), "static_fields");
@@ -200,25 +208,27 @@ void ShallowWaterEngine::setState(const float *state3)
     if (!state3) throw Error(CEL_EINVAL, "null state");
     check(hipStreamSynchronize(stream_), "sync");
     const size_t N = (size_t)X_ * Y_;
+    const size_t e = sizeof(real);
+    const std::vector<real> st(state3, state3 + 2 * N);
     for (int k = 0; k < 2; ++k) {
-        check(hipMemcpy2DAsync(w_[k], (size_t)P_ * 4, state3, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+        check(hipMemcpy2DAsync(w_[k], (size_t)P_ * e, st.data(), (size_t)X_ * e, (size_t)X_ * e, Y_,
                                hipMemcpyHostToDevice, stream_), "upload w");
-        check(hipMemcpy2DAsync(hu_[k], (size_t)P_ * 4, state3 + N, (size_t)X_ * 4, (size_t)X_ * 4, Y_,
+        check(hipMemcpy2DAsync(hu_[k], (size_t)P_ * e, st.data() + N, (size_t)X_ * e, (size_t)X_ * e, Y_,
                                hipMemcpyHostToDevice, stream_), "upload hu");
     }
-    std::vector<float> t(N);
+    std::vector<real> t(N);
     const float *hv = state3 + 2 * N;
     for (int y = 0; y < Y_; ++y)
         for (int x = 0; x < X_; ++x) t[(size_t)x * Y_ + y] = hv[(size_t)y * X_ + x];
     for (int k = 0; k < 2; ++k)
-        check(hipMemcpy2DAsync(hvT_[k], (size_t)PT_ * 4, t.data(), (size_t)Y_ * 4, (size_t)Y_ * 4, X_,
+        check(hipMemcpy2DAsync(hvT_[k], (size_t)PT_ * e, t.data(), (size_t)Y_ * e, (size_t)Y_ * e, X_,
                                hipMemcpyHostToDevice, stream_), "upload hv");
     // history textures start at zero (engine.cpp:3024-3034 creates them without data)
-    const size_t plane = (size_t)P_ * Y_ * sizeof(float);
+    const size_t plane = (size_t)P_ * Y_ * e;
     for (auto &g : G_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
     for (auto &g : FG_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
     if (comp_) {
-        const size_t tplane = (size_t)PT_ * X_ * sizeof(float);
+        const size_t tplane = (size_t)PT_ * X_ * e;
         for (int k = 0; k < 2; ++k) {
             check(hipMemsetAsync(w_lo_[k], 0, plane, stream_), "zero low words");
             check(hipMemsetAsync(hu_lo_[k], 0, plane, stream_), "zero low words");
@@ -229,42 +239,41 @@ void ShallowWaterEngine::setState(const float *state3)
     resetScheme();
 }
 
+// Device planes (w, hu row-major, hv transposed) of one buffer set -> three host planes.
+static void download_state(hipStream_t st, real *const w, real *const hu, real *const hvT, int X, int Y, int P, int PT,
+                           std::vector<real> &out)
+{
+    const size_t N = (size_t)X * Y, e = sizeof(real);
+    out.assign(3 * N, real(0));
+    std::vector<real> t(N);
+    hipError_t err = hipMemcpy2DAsync(out.data(), (size_t)X * e, w, (size_t)P * e, (size_t)X * e, Y,
+                                      hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess)
+        err = hipMemcpy2DAsync(out.data() + N, (size_t)X * e, hu, (size_t)P * e, (size_t)X * e, Y,
+                               hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess)
+        err = hipMemcpy2DAsync(t.data(), (size_t)Y * e, hvT, (size_t)PT * e, (size_t)Y * e, X, hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) throw Error(CEL_EDEVICE, std::string("download state: ") + hipGetErrorString(err));
+    for (int y = 0; y < Y; ++y)
+        for (int x = 0; x < X; ++x) out[2 * N + (size_t)y * X + x] = t[(size_t)x * Y + y];
+}
+
 void ShallowWaterEngine::getState(float *state3)
 {
     if (!state3) throw Error(CEL_EINVAL, "null state");
-    const size_t N = (size_t)X_ * Y_;
-    check(hipMemcpy2DAsync(state3, (size_t)X_ * 4, w_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
-                           hipMemcpyDeviceToHost, stream_), "download w");
-    check(hipMemcpy2DAsync(state3 + N, (size_t)X_ * 4, hu_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
-                           hipMemcpyDeviceToHost, stream_), "download hu");
-    std::vector<float> t(N);
-    check(hipMemcpy2DAsync(t.data(), (size_t)Y_ * 4, hvT_[cur_], (size_t)PT_ * 4, (size_t)Y_ * 4, X_,
-                           hipMemcpyDeviceToHost, stream_), "download hv");
-    check(hipStreamSynchronize(stream_), "download sync");
-    float *hv = state3 + 2 * N;
-    for (int y = 0; y < Y_; ++y)
-        for (int x = 0; x < X_; ++x) hv[(size_t)y * X_ + x] = t[(size_t)x * Y_ + y];
+    std::vector<real> st;
+    download_state(stream_, w_[cur_], hu_[cur_], hvT_[cur_], X_, Y_, P_, PT_, st);
+    for (size_t i = 0; i < st.size(); ++i) state3[i] = (float)st[i];
 }
 
 void ShallowWaterEngine::getStateF64(double *state3)
 {
     if (!state3) throw Error(CEL_EINVAL, "null state");
-    const size_t N = (size_t)X_ * Y_;
-    std::vector<float> hi(3 * N), lo(3 * N, 0.f);
-    getState(hi.data());
-    if (comp_) {
-        check(hipMemcpy2DAsync(lo.data(), (size_t)X_ * 4, w_lo_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
-                               hipMemcpyDeviceToHost, stream_), "download w_lo");
-        check(hipMemcpy2DAsync(lo.data() + N, (size_t)X_ * 4, hu_lo_[cur_], (size_t)P_ * 4, (size_t)X_ * 4, Y_,
-                               hipMemcpyDeviceToHost, stream_), "download hu_lo");
-        std::vector<float> t(N);
-        check(hipMemcpy2DAsync(t.data(), (size_t)Y_ * 4, hvT_lo_[cur_], (size_t)PT_ * 4, (size_t)Y_ * 4, X_,
-                               hipMemcpyDeviceToHost, stream_), "download hv_lo");
-        check(hipStreamSynchronize(stream_), "download sync");
-        for (int y = 0; y < Y_; ++y)
-            for (int x = 0; x < X_; ++x) lo[2 * N + (size_t)y * X_ + x] = t[(size_t)x * Y_ + y];
-    }
-    for (size_t i = 0; i < 3 * N; ++i) state3[i] = (double)hi[i] + (double)lo[i];
+    std::vector<real> hi, lo;
+    download_state(stream_, w_[cur_], hu_[cur_], hvT_[cur_], X_, Y_, P_, PT_, hi);
+    if (comp_) download_state(stream_, w_lo_[cur_], hu_lo_[cur_], hvT_lo_[cur_], X_, Y_, P_, PT_, lo);
+    for (size_t i = 0; i < hi.size(); ++i) state3[i] = (double)hi[i] + (comp_ ? (double)lo[i] : 0.0);
 }
 
 void ShallowWaterEngine::freeSideTables(int side)
@@ -372,7 +381,7 @@ void ShallowWaterEngine::buildSideTables(int side)
     const bool xs = side == CEL_WEST || side == CEL_EAST;
     const int nlines = xs ? Y_ : X_;
     const int ncell = wdt * nlines;
-    const float g = s_.gravity, sl = b.water_level;
+    const real g = s_.gravity, sl = b.water_level;
     // strip cell (strip index s, line position ln) -> grid cell
     auto cell_xy = [&](int s, int ln, int &x, int &y) {
         switch (side) {
@@ -385,107 +394,107 @@ void ShallowWaterEngine::buildSideTables(int side)
     auto BAat = [&](int x, int y) { return hostBA_[(size_t)y * X_ + x]; };
     if (ncell == 0) return;
     if (b.type == CEL_SPONGE) {
-        std::vector<float> gam(ncell);
-        const float del = xs ? dx_ : dy_;
-        const float Lw = (float)wdt * del;
+        std::vector<real> gam(ncell);
+        const real del = xs ? dx_ : dy_;
+        const real Lw = (real)wdt * del;
         for (int s = 0; s < wdt; ++s)
             for (int ln = 0; ln < nlines; ++ln) {
                 int x, y;
                 cell_xy(s, ln, x, y);
-                const float d_here = std::max(kMinDepth, sl - BAat(x, y));
-                float dist;
+                const real d_here = std::max(kMinDepth, sl - BAat(x, y));
+                real dist;
                 switch (side) {
-                case CEL_WEST: dist = (float)x * del; break;
-                case CEL_SOUTH: dist = (float)y * del; break;
-                case CEL_EAST: dist = (float)(X_ - x) * del; break;
-                default: dist = (float)(Y_ - y) * del; break;
+                case CEL_WEST: dist = (real)x * del; break;
+                case CEL_SOUTH: dist = (real)y * del; break;
+                case CEL_EAST: dist = (real)(X_ - x) * del; break;
+                default: dist = (real)(Y_ - y) * del; break;
                 }
                 gam[(size_t)s * nlines + ln] = sponge_function(PI_, Lw, dist) /
                     sponge_function(PI_, Lw, dist + std::sqrt(g * d_here) * current_timestep_);
             }
-        check(hipMalloc(&sd.d_tab, gam.size() * 4), "hipMalloc(sponge)");
-        check(hipMemcpy(sd.d_tab, gam.data(), gam.size() * 4, hipMemcpyHostToDevice), "upload sponge");
-        table_bytes_ += gam.size() * 4;
+        check(hipMalloc(&sd.d_tab, gam.size() * sizeof(real)), "hipMalloc(sponge)");
+        check(hipMemcpy(sd.d_tab, gam.data(), gam.size() * sizeof(real), hipMemcpyHostToDevice), "upload sponge");
+        table_bytes_ += gam.size() * sizeof(real);
     } else if (b.type == CEL_SINEWAVE) {
-        std::vector<float> tab((size_t)ncell * 5, 0.f);
-        const float omega = 2 * PI_ / b.period;
+        std::vector<real> tab((size_t)ncell * 5, real(0));
+        const real omega = 2 * PI_ / b.period;
         sd.sine_omega = omega;
         for (int s = 0; s < wdt; ++s)
             for (int ln = 0; ln < nlines; ++ln) {
                 int x, y;
                 cell_xy(s, ln, x, y);
-                const float d_here = std::max(0.f, sl - BAat(x, y));
-                float px, py;
+                const real d_here = std::max(real(0), sl - BAat(x, y));
+                real px, py;
                 switch (side) {
-                case CEL_EAST: px = (float)(x - X_) * dx_; py = (float)y * dy_; break;
-                case CEL_NORTH: px = (float)x * dx_; py = (float)(y - Y_) * dy_; break;
-                default: px = (float)x * dx_; py = (float)y * dy_; break;
+                case CEL_EAST: px = (real)(x - X_) * dx_; py = (real)y * dy_; break;
+                case CEL_NORTH: px = (real)x * dx_; py = (real)(y - Y_) * dy_; break;
+                default: px = (real)x * dx_; py = (real)y * dy_; break;
                 }
-                float *t = &tab[((size_t)s * nlines + ln) * 5];
+                real *t = &tab[((size_t)s * nlines + ln) * 5];
                 if (d_here > kMinDepth) {
-                    const float k = wavenumber(omega, d_here, g);
-                    const float c = omega / k;
-                    t[0] = std::cos(b.theta) * px * k;
-                    t[1] = std::sin(b.theta) * py * k;
-                    t[2] = c * std::cos(b.theta);
-                    t[3] = c * std::sin(b.theta);
+                    const real k = wavenumber(omega, d_here, g);
+                    const real c = omega / k;
+                    t[0] = std::cos((real)b.theta) * px * k;
+                    t[1] = std::sin((real)b.theta) * py * k;
+                    t[2] = c * std::cos((real)b.theta);
+                    t[3] = c * std::sin((real)b.theta);
                     t[4] = 1.f;
                 }
             }
-        check(hipMalloc(&sd.d_tab, tab.size() * 4), "hipMalloc(sine)");
-        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice), "upload sine");
-        table_bytes_ += tab.size() * 4;
+        check(hipMalloc(&sd.d_tab, tab.size() * sizeof(real)), "hipMalloc(sine)");
+        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * sizeof(real), hipMemcpyHostToDevice), "upload sine");
+        table_bytes_ += tab.size() * sizeof(real);
     } else if (b.type == CEL_IRREGULARWAVES) {
         const int L = xs ? Lx_ : Ly_;
         const int pn = 1 << (L - 1);
         const int nact = std::min(b.n_waves, pn);
         sd.irr_nw = nact;
-        std::vector<float> wave((size_t)std::max(nact, 1) * 3, 0.f);
+        std::vector<real> wave((size_t)std::max(nact, 1) * 3, real(0));
         for (int i = 0; i < nact; ++i) {
             wave[i] = b.waves[4 * i];
             wave[std::max(nact, 1) + i] = 2 * PI_ / b.waves[4 * i + 1];
             wave[2 * std::max(nact, 1) + i] = b.waves[4 * i + 3];
         }
-        std::vector<float> tab((size_t)ncell * std::max(nact, 1) * 4, 0.f);
+        std::vector<real> tab((size_t)ncell * std::max(nact, 1) * 4, real(0));
         std::vector<unsigned char> wet(ncell, 0);
         for (int s = 0; s < wdt; ++s)
             for (int ln = 0; ln < nlines; ++ln) {
                 const size_t cell = (size_t)s * nlines + ln;
                 if (ln < 2 || ln >= nlines - 2) continue;
                 int col_or_row = xs ? (side == CEL_WEST ? s : X_ - wdt + s) : (side == CEL_SOUTH ? s : Y_ - wdt + s);
-                float px, py, Bh;
+                real px, py, Bh;
                 if (xs) {
-                    px = side == CEL_WEST ? (float)col_or_row * dx_ : (float)(col_or_row - X_) * dx_;
-                    py = (float)ln * dy_;
+                    px = side == CEL_WEST ? (real)col_or_row * dx_ : (real)(col_or_row - X_) * dx_;
+                    py = (real)ln * dy_;
                     Bh = BAat(col_or_row, ln);
                 } else {
-                    px = (float)ln * dx_;
-                    py = side == CEL_SOUTH ? (float)col_or_row * dy_ : (float)(col_or_row - Y_) * dy_;
+                    px = (real)ln * dx_;
+                    py = side == CEL_SOUTH ? (real)col_or_row * dy_ : (real)(col_or_row - Y_) * dy_;
                     Bh = BAat(ln, col_or_row);
                 }
-                const float d_here = std::max(0.f, sl - Bh);
+                const real d_here = std::max(real(0), sl - Bh);
                 wet[cell] = d_here > kMinDepth;
                 if (!wet[cell]) continue;
                 for (int i = 0; i < nact; ++i) {
-                    const float theta = b.waves[4 * i + 2];
-                    const float omega = 2 * PI_ / b.waves[4 * i + 1];
-                    const float k = wavenumber(omega, d_here, g);
-                    const float c = omega / k;
-                    float *t = &tab[(cell * nact + i) * 4];
+                    const real theta = b.waves[4 * i + 2];
+                    const real omega = 2 * PI_ / b.waves[4 * i + 1];
+                    const real k = wavenumber(omega, d_here, g);
+                    const real c = omega / k;
+                    real *t = &tab[(cell * nact + i) * 4];
                     t[0] = std::cos(theta) * px * k;
                     t[1] = std::sin(theta) * py * k;
                     t[2] = c * std::cos(theta);
                     t[3] = c * std::sin(theta);
                 }
             }
-        check(hipMalloc(&sd.d_tab, tab.size() * 4), "hipMalloc(irregular)");
-        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice), "upload irregular");
-        check(hipMalloc(&sd.d_irr_wave, wave.size() * 4), "hipMalloc(irregular waves)");
-        check(hipMemcpy(sd.d_irr_wave, wave.data(), wave.size() * 4, hipMemcpyHostToDevice), "upload waves");
+        check(hipMalloc(&sd.d_tab, tab.size() * sizeof(real)), "hipMalloc(irregular)");
+        check(hipMemcpy(sd.d_tab, tab.data(), tab.size() * sizeof(real), hipMemcpyHostToDevice), "upload irregular");
+        check(hipMalloc(&sd.d_irr_wave, wave.size() * sizeof(real)), "hipMalloc(irregular waves)");
+        check(hipMemcpy(sd.d_irr_wave, wave.data(), wave.size() * sizeof(real), hipMemcpyHostToDevice), "upload waves");
         check(hipMalloc(&sd.d_wet, wet.size()), "hipMalloc(wet)");
         check(hipMemcpy(sd.d_wet, wet.data(), wet.size(), hipMemcpyHostToDevice), "upload wet");
-        check(hipMalloc(&sd.d_irr_out, (size_t)ncell * 3 * 4), "hipMalloc(irregular out)");
-        table_bytes_ += tab.size() * 4 + wave.size() * 4 + wet.size() + (size_t)ncell * 12;
+        check(hipMalloc(&sd.d_irr_out, (size_t)ncell * 3 * sizeof(real)), "hipMalloc(irregular out)");
+        table_bytes_ += tab.size() * sizeof(real) + wave.size() * sizeof(real) + wet.size() + (size_t)ncell * 3 * sizeof(real);
     }
 }
 
@@ -503,19 +512,19 @@ void ShallowWaterEngine::resetTimestep(float dt)
 }
 
 // timeSeriesHelper, engine.cpp:1893-1912
-void ShallowWaterEngine::timeSeries(int side, float t, float &eta, float &hu, float &hv)
+void ShallowWaterEngine::timeSeries(int side, real t, real &eta, real &hu, real &hv)
 {
     SideState &sd = side_[side];
     const int n = (int)sd.uts.size() / 4;
     int &it = sd.uts_iter;
-    eta = 0.f; hu = 0.f; hv = 0.f;
-    while (it < n && sd.uts[4 * it] < t) ++it;
+    eta = 0; hu = 0; hv = 0;
+    while (it < n && (real)sd.uts[4 * it] < t) ++it;
     if (it >= n) return;
     const float *a = &sd.uts[4 * (it - 1)], *b = &sd.uts[4 * it];
-    const float m = (t - a[0]) / (b[0] - a[0]);
-    eta = a[1] + (b[1] - a[1]) * m;
-    hu = a[2] + (b[2] - a[2]) * m;
-    hv = a[3] + (b[3] - a[3]) * m;
+    const real m = (t - (real)a[0]) / ((real)b[0] - (real)a[0]);
+    eta = (real)a[1] + ((real)b[1] - (real)a[1]) * m;
+    hu = (real)a[2] + ((real)b[2] - (real)a[2]) * m;
+    hv = (real)a[3] + ((real)b[3] - (real)a[3]) * m;
 }
 
 void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
@@ -533,23 +542,23 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     p.w_new = w_[n]; p.Dx = Dx_; p.DyT = DyT_;
     p.w_lo = w_lo_[c]; p.w_lo_new = w_lo_[n];
     // fillConstantBuffers, engine.cpp:4099-4136
-    const float g = s_.gravity;
-    p.two_theta = 2 * s_.theta;
+    const real g = s_.gravity;
+    p.two_theta = real(2) * (real)s_.theta;
     p.g = g;
-    p.half_g = 0.5f * g;
-    p.Bcoef_g = (1 / 15.0f) * g;
+    p.half_g = real(0.5) * g;
+    p.Bcoef_g = (real(1) / real(15)) * g;
     p.g_over_dx = g / dx_;
     p.g_over_dy = g / dy_;
-    p.one_over_dx = 1.0f / dx_;
-    p.one_over_dy = 1.0f / dy_;
+    p.one_over_dx = real(1) / dx_;
+    p.one_over_dy = real(1) / dy_;
     p.dt = current_timestep_;
-    const float m = std::max(dx_, dy_);
-    p.epsilon = std::min(s_.epsilon, (m * m) * (m * m));   // CalcEpsilon, engine.cpp:290-299
+    const real m = std::max(dx_, dy_);
+    p.epsilon = std::min((real)s_.epsilon, (m * m) * (m * m));   // CalcEpsilon, engine.cpp:290-299
     p.friction = s_.friction;
     p.sea_level = s_.still_water_elevation;
     p.is_manning = s_.is_manning;
     // AdamsBashforthCoef and SecondOrderDerivatives weights (compute.hlsl:560-605)
-    const float dt = current_timestep_, d1 = old_timestep_, d2 = old_old_timestep_;
+    const real dt = current_timestep_, d1 = old_timestep_, d2 = old_old_timestep_;
     p.ab[0] = (dt / d1) * ((2 * dt + 6 * d1 + 3 * d2) / (d1 + d2)) + 6;
     p.ab[1] = -(dt / d1) * ((2 * dt + 3 * d1 + 3 * d2) / (d2));
     p.ab[2] = (dt / d2) * ((2 * dt + 3 * d1) / (d1 + d2));
@@ -573,13 +582,13 @@ void ShallowWaterEngine::fillStatic(StaticParams &p) const
     p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
     p.BA = BA_;
     p.Kx = Kx_; p.Qx = Qx_; p.Ky = Ky_; p.Qy = Qy_; p.DDx = DDx_; p.DDy = DDy_; p.KyT = KyT_; p.QyT = QyT_;
-    p.kcoef = 1 / 15.0f + 1 / 3.0f;
+    p.kcoef = real(1) / real(15) + real(1) / real(3);
     p.dx2 = dx_ * dx_;
     p.dy2 = dy_ * dy_;
-    p.coef_sl = s_.is_boussinesq ? s_.still_water_elevation : -9999.f;
+    p.coef_sl = s_.is_boussinesq ? (real)s_.still_water_elevation : real(-9999);
     p.sea_level = s_.still_water_elevation;
-    p.one_over_dx = 1.0f / dx_;
-    p.one_over_dy = 1.0f / dy_;
+    p.one_over_dx = real(1) / dx_;
+    p.one_over_dy = real(1) / dy_;
 }
 
 void ShallowWaterEngine::fillLines(LineParams &p) const
@@ -612,15 +621,15 @@ void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
     // host state the plan moves on (restored when only peeking)
     const int keep_i[8] = {cur_, ring_old_, ring_oo_, ring_scr_, time_scheme_now_, timestep_count_,
                            correction_steps_count_, 0};
-    const float keep_f[4] = {total_time_, current_timestep_, old_timestep_, old_old_timestep_};
+    const real keep_f[4] = {total_time_, current_timestep_, old_timestep_, old_old_timestep_};
     int keep_uts[4];
     for (int s = 0; s < 4; ++s) keep_uts[s] = side_[s].uts_iter;
     fillExplicit(L.ep);
     // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
     // steps nothing ever reads them
     if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
-        const size_t plane = (size_t)P_ * Y_ * 4, tplane = (size_t)PT_ * X_ * 4;
-        auto add = [&](float *d, const float *src, size_t bytes) { L.copy[L.ncopy++] = StepLaunch::Copy{d, src, bytes}; };
+        const size_t plane = (size_t)P_ * Y_ * sizeof(real), tplane = (size_t)PT_ * X_ * sizeof(real);
+        auto add = [&](real *d, const real *src, size_t bytes) { L.copy[L.ncopy++] = StepLaunch::Copy{d, src, bytes}; };
         for (int k = 0; k < 3; ++k) add(Gp_[k], G_[ring_scr_][k], plane);
         for (int k = 0; k < 2; ++k) add(FGp_[k], FG_[ring_scr_][k], plane);
         add(Cw_, w_[cur_], plane);
@@ -709,7 +718,7 @@ void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
 
 // Issues one planned step on the stream.  With dclock set, the kernels take t^n and the
 // time-series values from that device entry instead of their parameter blocks.
-void ShallowWaterEngine::issueStep(const StepLaunch &L, const float *dclock, bool profile)
+void ShallowWaterEngine::issueStep(const StepLaunch &L, const real *dclock, bool profile)
 {
     hipEvent_t pe[4] = {};
     if (profile) {
@@ -758,15 +767,15 @@ void ShallowWaterEngine::setGraphSteps(int k)
     clearGraphs();
     if (k > 0 && !d_clock_) {
         void *mem = nullptr;
-        check(hipMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(float)), "hipMalloc(clock)");
-        d_clock_ = static_cast<float *>(mem);
+        check(hipMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(real)), "hipMalloc(clock)");
+        d_clock_ = static_cast<real *>(mem);
         for (auto &h : h_clock_) {
-            check(hipHostMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(float), hipHostMallocDefault),
+            check(hipHostMalloc(&mem, (size_t)kMaxGraphSteps * kClockStride * sizeof(real), hipHostMallocDefault),
                   "hipHostMalloc(clock)");
-            h = static_cast<float *>(mem);
+            h = static_cast<real *>(mem);
         }
         for (auto &e : clock_ev_) check(hipEventCreate(&e), "hipEventCreate");
-        table_bytes_ += (size_t)kMaxGraphSteps * kClockStride * sizeof(float);
+        table_bytes_ += (size_t)kMaxGraphSteps * kClockStride * sizeof(real);
     }
     graph_steps_ = k;
 }
@@ -815,10 +824,10 @@ void ShallowWaterEngine::graphBlock()
         graphs_.push_back(GraphEntry{key, exec});
     }
     // the host clock buffer of this slot was last read by the copy two blocks ago
-    float *h = h_clock_[clock_slot_];
+    real *h = h_clock_[clock_slot_];
     if (clock_used_[clock_slot_]) check(hipEventSynchronize(clock_ev_[clock_slot_]), "clock event");
     for (int k = 0; k < K; ++k) std::memcpy(h + (size_t)k * kClockStride, plans[k].clock, sizeof(plans[k].clock));
-    check(hipMemcpyAsync(d_clock_, h, (size_t)K * kClockStride * sizeof(float), hipMemcpyHostToDevice, stream_),
+    check(hipMemcpyAsync(d_clock_, h, (size_t)K * kClockStride * sizeof(real), hipMemcpyHostToDevice, stream_),
           "upload clock");
     check(hipEventRecord(clock_ev_[clock_slot_], stream_), "clock event");
     clock_used_[clock_slot_] = true;
@@ -904,10 +913,10 @@ void ShallowWaterEngine::runStats(double *out9)
     p.nblocks = stats_blocks_;
     p.w = w_[cur_]; p.hu = hu_[cur_]; p.hvT = hvT_[cur_]; p.BA = BA_;
     p.g = s_.gravity;
-    p.one_over_dx = 1.0f / dx_;
-    p.one_over_dy = 1.0f / dy_;
-    const float m = std::max(dx_, dy_);
-    p.epsilon = std::min(s_.epsilon, (m * m) * (m * m));
+    p.one_over_dx = real(1) / dx_;
+    p.one_over_dy = real(1) / dy_;
+    const real m = std::max(dx_, dy_);
+    p.epsilon = std::min((real)s_.epsilon, (m * m) * (m * m));
     p.partial = d_stats_;
     p.out = d_stats_ + (size_t)stats_blocks_ * kStatsValues;
     check(launch_stats(p, stream_), "stats");
@@ -934,7 +943,7 @@ void ShallowWaterEngine::deriveStats(const double *r, cel_stats *o) const
     o->max_speed = std::sqrt(r[5]);
     o->max_depth = r[6];
     o->max_froude = std::sqrt(r[8] / g);
-    o->timestep = current_timestep_;
+    o->timestep = (float)current_timestep_;
     o->cfl_number = r[7] * (double)current_timestep_;
     o->cells = 16 * (nx_ / 4) * (ny_ / 4);
 }
@@ -958,7 +967,7 @@ void ShallowWaterEngine::resetTimestepAdaptive(const cel_adaptive &a, cel_stats 
         target_cfl_set_ = true;
     }
     const float safety = target_cfl_set_ ? target_cfl_ : a.target_cfl;
-    float cur = current_timestep_;
+    float cur = (float)current_timestep_;
     if (!dt_initialized_) {
         cur = s_.timestep;
         cfl = safety / cur;
@@ -966,8 +975,9 @@ void ShallowWaterEngine::resetTimestepAdaptive(const cel_adaptive &a, cel_stats 
         dt_initialized_ = true;
     }
     if (time_scheme_now_ == CEL_PREDICTOR_ADAPTIVE) {
+        const float old = (float)old_timestep_;
         const float target = std::max(safety / cfl, s_.timestep * a.min_timestep_ratio);
-        cur = target > old_timestep_ ? a.ema_alpha * target + (1 - a.ema_alpha) * old_timestep_ : target;
+        cur = target > old ? a.ema_alpha * target + (1 - a.ema_alpha) * old : target;
     } else {
         cur = uniform_dt_;
     }
@@ -1028,19 +1038,24 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
         check(hipEventElapsedTime(&ms, ev_[6], ev_[7]), "event");
         return 1000.f * ms / repeats;
     };
-    const int keep_rows = rows_;
-    float best = 0.f;
-    int best_rows = rows_;
-    for (int rows : {4, 8, 16}) {
-        rows_ = rows;
-        ExplicitParams p;
-        fillExplicit(p);
-        const float us = time_it([&] { check(launch_explicit(scheme, comp_, rows, p, stream_), "explicit_step"); });
-        if (rows == 4 || us < best) { best = us; best_rows = rows; }
-    }
-    rows_ = best_rows;
-    tuned_us_[0] = best;
-    (void)keep_rows;
+    // candidates are timed in three interleaved rounds and judged by their fastest round,
+    // so clock ramps and other transients do not pick the winner
+    constexpr int kRounds = 3;
+    const int rows_cand[3] = {4, 8, 16};
+    float rows_us[3] = {-1.f, -1.f, -1.f};
+    for (int round = 0; round < kRounds; ++round)
+        for (int c = 0; c < 3; ++c) {
+            rows_ = rows_cand[c];
+            ExplicitParams p;
+            fillExplicit(p);
+            const float us = time_it([&] { check(launch_explicit(scheme, comp_, rows_, p, stream_), "explicit_step"); });
+            if (rows_us[c] < 0.f || us < rows_us[c]) rows_us[c] = us;
+        }
+    int bc = 0;
+    for (int c = 1; c < 3; ++c)
+        if (rows_us[c] < rows_us[bc]) bc = c;
+    rows_ = rows_cand[bc];
+    tuned_us_[0] = rows_us[bc];
     LineParams lp;
     fillLines(lp);
     IrregularAll none;
@@ -1049,17 +1064,21 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
         const int n = dir == 0 ? nx_ : ny_;
         LineParams q = lp;
         if (dir == 0) q.nlines_y = 0; else q.nlines_x = 0;
-        int best_shape = dir == 0 ? shape_x_ : shape_y_;
-        float best_us = -1.f;
-        for (int sh = 0; sh < kNumShapes; ++sh) {
-            const int cap = shape_capacity(sh);
-            if (cap < n || cap > std::max(4 * n, 64)) continue;
-            const int sx = dir == 0 ? sh : shape_x_, sy = dir == 0 ? shape_y_ : sh;
-            const float us = time_it([&] { check(launch_solve(comp_, sx, sy, q, none, stream_), "solve_lines"); });
-            if (best_us < 0.f || us < best_us) { best_us = us; best_shape = sh; }
-        }
+        float shape_us[kNumShapes];
+        for (auto &v : shape_us) v = -1.f;
+        for (int round = 0; round < kRounds; ++round)
+            for (int sh = 0; sh < kNumShapes; ++sh) {
+                const int cap = shape_capacity(sh);
+                if (cap < n || cap > std::max(4 * n, 64)) continue;
+                const int sx = dir == 0 ? sh : shape_x_, sy = dir == 0 ? shape_y_ : sh;
+                const float us = time_it([&] { check(launch_solve(comp_, sx, sy, q, none, stream_), "solve_lines"); });
+                if (shape_us[sh] < 0.f || us < shape_us[sh]) shape_us[sh] = us;
+            }
+        int best_shape = -1;
+        for (int sh = 0; sh < kNumShapes; ++sh)
+            if (shape_us[sh] >= 0.f && (best_shape < 0 || shape_us[sh] < shape_us[best_shape])) best_shape = sh;
         if (dir == 0) shape_x_ = best_shape; else shape_y_ = best_shape;
-        tuned_us_[1 + dir] = best_us;
+        tuned_us_[1 + dir] = shape_us[best_shape];
     }
     if (out) getTuning(out);
 }
@@ -1069,13 +1088,14 @@ void ShallowWaterEngine::coefficients(float *ax, float *ay)
     StaticParams sp;
     fillStatic(sp);
     const size_t N = (size_t)X_ * Y_;
-    float *d = nullptr;
-    check(hipMalloc(&d, N * 6 * 4), "hipMalloc");
+    real *d = nullptr;
+    check(hipMalloc(&d, N * 6 * sizeof(real)), "hipMalloc");
+    std::vector<real> h(N * 6);
     check(launch_export_coefficients(sp, d, d + 3 * N, stream_), "export_coefficients");
-    check(hipMemcpyAsync(ax, d, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
-    check(hipMemcpyAsync(ay, d + 3 * N, N * 3 * 4, hipMemcpyDeviceToHost, stream_), "download");
+    check(hipMemcpyAsync(h.data(), d, N * 6 * sizeof(real), hipMemcpyDeviceToHost, stream_), "download");
     check(hipStreamSynchronize(stream_), "sync");
     (void)hipFree(d);
+    for (size_t i = 0; i < 3 * N; ++i) { ax[i] = (float)h[i]; ay[i] = (float)h[3 * N + i]; }
 }
 
-}  // namespace cel
+}  // namespace CEL_NS

@@ -1,12 +1,11 @@
 // engine.hpp -- host side of the MI355X Celeris step: ShallowWaterEngine.
 //
 // Mirrors the numerical interface of the reference class (engine.hpp:58-97):
-// construction + remesh, resetTimestep (fixed or caller-chosen dt), timestep(),
-// boundary changes (createBoundaryShaders), plus state upload/download.  All device
-// resources are owned here; the D3D11 device/context pair of the reference is replaced
-// by one HIP stream per engine.
-#pragma once
-
+// construction + remesh, resetTimestep (fixed, caller-chosen or adaptive dt),
+// timestep(), boundary changes (createBoundaryShaders), plus state upload/download.  All
+// device resources are owned here; the D3D11 device/context pair of the reference is
+// replaced by one HIP stream per engine.  Included once per working precision (see
+// params.hpp): cel::ShallowWaterEngine works in FP32, cel64::ShallowWaterEngine in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,14 +14,21 @@
 #include <vector>
 
 #include "../../include/celeris.h"
+#include "engine_api.hpp"
 #include "params.hpp"
 
-namespace cel {
+#if (defined(CEL_FP64) && !defined(CEL_ENGINE_HPP_FP64)) || (!defined(CEL_FP64) && !defined(CEL_ENGINE_HPP_FP32))
+#ifdef CEL_FP64
+#define CEL_ENGINE_HPP_FP64
+#else
+#define CEL_ENGINE_HPP_FP32
+#endif
 
-struct Error : std::runtime_error {
-    int status;
-    Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
-};
+namespace CEL_NS {
+
+#ifdef CEL_FP64
+using cel::Error;
+#endif
 
 hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st);
 hipError_t launch_solve(bool comp, int shape_x, int shape_y, const LineParams &p, const IrregularAll &irr,
@@ -31,7 +37,7 @@ int default_shape(int n);
 int shape_capacity(int shape);
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
-hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st);
+hipError_t launch_export_coefficients(const StaticParams &p, real *ax, real *ay, hipStream_t st);
 hipError_t launch_stats(const StatsParams &p, hipStream_t st);
 
 // st_getCountOfMatrices, engine.cpp:741-748
@@ -43,17 +49,17 @@ struct SideState {
     std::vector<float> waves;     // irregular table (amp, period, theta, phase)
     std::vector<float> uts;       // (t, eta, hu, hv)
     int uts_iter = 1;             // UniformTimeSeries::iter
-    float *d_tab = nullptr;       // sponge gamma / sine table / irregular table
-    float *d_irr_out = nullptr;   // irregular strip values
-    float *d_irr_wave = nullptr;  // amp, omega, phase per component
+    real *d_tab = nullptr;        // sponge gamma / sine table / irregular table
+    real *d_irr_out = nullptr;    // irregular strip values
+    real *d_irr_wave = nullptr;   // amp, omega, phase per component
     unsigned char *d_wet = nullptr;
-    float sine_omega = 0.f;
+    real sine_omega = 0;
     int irr_nw = 0;
 };
 
 // Everything one step issues (kernel parameter blocks, predictor copies, step clock).
 struct StepLaunch {
-    struct Copy { float *dst; const float *src; size_t bytes; };
+    struct Copy { real *dst; const real *src; size_t bytes; };
     int scheme;
     int rows, shape_x, shape_y;     // explicit band rows, line-solve shapes
     int ncopy;
@@ -62,46 +68,47 @@ struct StepLaunch {
     LineParams lp;
     IrregularAll irr;
     BoundaryParams bp;
-    float clock[kClockStride];
+    real clock[kClockStride];
 };
 
 constexpr int kMaxGraphSteps = 1024;
 
-class ShallowWaterEngine {
+class ShallowWaterEngine final : public cel::EngineApi {
 public:
     ShallowWaterEngine(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3, const float *state3);
     ~ShallowWaterEngine();
     ShallowWaterEngine(const ShallowWaterEngine &) = delete;
     ShallowWaterEngine &operator=(const ShallowWaterEngine &) = delete;
 
-    void setBoundary(int side, const cel_boundary &b);       // createBoundaryShaders()
-    void resetTimestep(float dt);                             // resetTimestep() with a fixed dt
-    void timestep();                                          // ShallowWaterEngine::timestep()
-    void step(int n);
-    void sync();
-    void getState(float *state3);
-    void getStateF64(double *state3);
-    void setState(const float *state3);
-    double time() const { return total_time_; }
-    int scheme() const { return time_scheme_now_; }
-    float timedSteps(int n);
-    void setProfiling(bool on);
-    bool kernelTime(const std::string &name, float *ms, int *launches);
-    size_t deviceBytes() const { return arena_bytes_ + table_bytes_; }
+    void setBoundary(int side, const cel_boundary &b) override;   // createBoundaryShaders()
+    void resetTimestep(float dt) override;                         // resetTimestep() with a fixed dt
+    void timestep();                                               // ShallowWaterEngine::timestep()
+    void step(int n) override;
+    void sync() override;
+    void getState(float *state3) override;
+    void getStateF64(double *state3) override;
+    void setState(const float *state3) override;
+    double time() const { return (double)total_time_; }
+    double timeValue() const override { return (double)total_time_; }
+    int scheme() const override { return time_scheme_now_; }
+    float timedSteps(int n) override;
+    void setProfiling(bool on) override;
+    bool kernelTime(const std::string &name, float *ms, int *launches) override;
+    size_t deviceBytes() const override { return arena_bytes_ + table_bytes_; }
     bool compensated() const { return comp_; }
     // device bytes an engine with these settings holds after creation (admission control)
     static size_t estimateBytes(const cel_settings &s, const cel_boundary bnd[4]);
-    void coefficients(float *ax, float *ay);
-    void stats(cel_stats *out);
-    void resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out);
-    void setGraphSteps(int k);                                // steps per captured block (0: off)
+    void coefficients(float *ax, float *ay) override;
+    void stats(cel_stats *out) override;
+    void resetTimestepAdaptive(const cel_adaptive &a, cel_stats *out) override;
+    void setGraphSteps(int k) override;                       // steps per captured block (0: off)
     // plans the next step (kernel parameters from the current host state) and, if advance,
     // moves the host state on as the step will
     void planStep(StepLaunch &L, bool advance = true);
     // tile shape and line-solve shapes: timed choice, current choice, caller's choice
-    void autotune(int repeats, cel_tuning *out);
-    void getTuning(cel_tuning *out) const;
-    void setTuning(const cel_tuning &t);
+    void autotune(int repeats, cel_tuning *out) override;
+    void getTuning(cel_tuning *out) const override;
+    void setTuning(const cel_tuning &t) override;
 
 private:
     void allocate();
@@ -114,7 +121,7 @@ private:
     void fillLines(LineParams &p) const;
     void fillStatic(StaticParams &p) const;
     void check(hipError_t e, const char *what) const;
-    void issueStep(const StepLaunch &L, const float *dclock, bool profile);
+    void issueStep(const StepLaunch &L, const real *dclock, bool profile);
 
     // captured multi-step blocks
     struct GraphKey {
@@ -124,37 +131,37 @@ private:
     struct GraphEntry { GraphKey key; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs_;
     int graph_steps_ = 0;
-    float *d_clock_ = nullptr;
-    float *h_clock_[2] = {};
+    real *d_clock_ = nullptr;
+    real *h_clock_[2] = {};
     hipEvent_t clock_ev_[2] = {};
     bool clock_used_[2] = {};
     int clock_slot_ = 0;
     GraphKey discreteKey() const;
     void graphBlock();
     void clearGraphs();
-    void timeSeries(int side, float t, float &eta, float &hu, float &hv);
+    void timeSeries(int side, real t, real &eta, real &hu, real &hv);
 
     cel_settings s_;
     SideState side_[4];
     int nx_, ny_, X_, Y_, P_, PT_, Lx_, Ly_;
-    float dx_, dy_, PI_;
+    real dx_, dy_, PI_;
     hipStream_t stream_ = nullptr;
     int device_ = 0;
 
     // device arena
     char *arena_ = nullptr;
     size_t arena_bytes_ = 0, table_bytes_ = 0;
-    float *w_[2], *hu_[2], *hvT_[2];
+    real *w_[2], *hu_[2], *hvT_[2];
     bool comp_ = false;                                       // compensated two-float state
-    float *w_lo_[2] = {}, *hu_lo_[2] = {}, *hvT_lo_[2] = {};
-    float *BY_, *BX_, *BA_;
-    float *Kx_, *Qx_, *Ky_, *Qy_, *DDx_, *DDy_, *KyT_, *QyT_;   // static per-cell fields
-    float *Dx_, *DyT_;
-    float *G_[3][3];          // ring slot, component
-    float *FG_[3][2];
-    float *Gp_[3], *FGp_[2];
-    float *Cw_, *Chu_, *ChvT_;
-    std::vector<float> hostBA_;
+    real *w_lo_[2] = {}, *hu_lo_[2] = {}, *hvT_lo_[2] = {};
+    real *BY_, *BX_, *BA_;
+    real *Kx_, *Qx_, *Ky_, *Qy_, *DDx_, *DDy_, *KyT_, *QyT_;   // static per-cell fields
+    real *Dx_, *DyT_;
+    real *G_[3][3];           // ring slot, component
+    real *FG_[3][2];
+    real *Gp_[3], *FGp_[2];
+    real *Cw_, *Chu_, *ChvT_;
+    std::vector<real> hostBA_;
 
     // GetStats buffers and resetTimestep's adaptive-dt state (engine.cpp:1698-1726)
     double *d_stats_ = nullptr;     // partials, then the nine results
@@ -176,8 +183,8 @@ private:
     int time_scheme_now_ = CEL_EULER;
     int timestep_count_ = 1;
     int correction_steps_count_ = 0;
-    float total_time_ = 0.f;
-    float current_timestep_, old_timestep_, old_old_timestep_;
+    real total_time_ = 0;
+    real current_timestep_, old_timestep_, old_old_timestep_;
 
     // profiling: five events per step recorded without host synchronisation and
     // reduced after the stream drains (collectProfile)
@@ -191,4 +198,6 @@ private:
     void collectProfile();
 };
 
-}  // namespace cel
+}  // namespace CEL_NS
+
+#endif

@@ -17,123 +17,152 @@
 //                     chain per cell (compute.hlsl:985-1194, engine.cpp:1345-1438).
 //
 // Arithmetic keeps the shader's operation order; the build disables FMA contraction so
-// that every product and sum rounds as written.
+// that every product and sum rounds as written.  The file is compiled twice: with
+// real = float (namespace cel, the reference's precision) and, with CEL_FP64 defined,
+// real = double (namespace cel64, the FP64 variant).
 #include <hip/hip_runtime.h>
 #include "params.hpp"
 
 
-namespace cel {
+namespace CEL_NS {
+
+// math in the working precision
+__device__ __forceinline__ float SQRT(float x) { return sqrtf(x); }
+__device__ __forceinline__ double SQRT(double x) { return sqrt(x); }
+__device__ __forceinline__ float SIN(float x) { return sinf(x); }
+__device__ __forceinline__ double SIN(double x) { return sin(x); }
+__device__ __forceinline__ float FMAX(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double FMAX(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float FMIN(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double FMIN(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float FABS(float x) { return fabsf(x); }
+__device__ __forceinline__ double FABS(double x) { return fabs(x); }
+__device__ __forceinline__ float POW(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double POW(double x, double y) { return pow(x, y); }
+
+// 16 bytes of consecutive elements (4 floats or 2 doubles) and a pair of elements, for
+// vector loads and stores
+constexpr int kVec = 16 / (int)sizeof(real);
+struct alignas(16) Vec { real v[kVec]; };
+struct alignas(2 * sizeof(real)) Pair { real x, y; };
 
 // ------------------------------------------------------------------ helpers
 
 // IEEE min/max (v_min_f32 / v_max_f32, fused into v_min3/v_max3).  They agree with the
 // shader's max/min on every non-NaN input up to the sign of a zero result, which never
 // changes a nonzero value downstream.
-__device__ __forceinline__ float fmaxr(float a, float b) { return fmaxf(a, b); }
-__device__ __forceinline__ float fminr(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ real fmaxr(real a, real b) { return FMAX(a, b); }
+__device__ __forceinline__ real fminr(real a, real b) { return FMIN(a, b); }
 
 // lane i receives lane i+1 / i-1 through a DPP wave shift (edge lanes keep their own
-// value; they are halo lanes)
-__device__ __forceinline__ float from_right(float v)
+// value; they are halo lanes); a double moves as its two 32-bit halves
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
 {
     const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));   // wave_shl:1
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float from_left(float v)
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
 {
-    const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));   // wave_shr:1
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = (int)(unsigned)(b & 0xffffffffull), hi = (int)(unsigned)(b >> 32);
+    const unsigned nlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const unsigned nhi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)nhi << 32) | nlo);
 }
+__device__ __forceinline__ real from_right(real v) { return dpp_move<0x130>(v); }   // wave_shl:1
+__device__ __forceinline__ real from_left(real v) { return dpp_move<0x138>(v); }    // wave_shr:1
 
 // MinMod, compute.hlsl:292-296: the minimum of three positive slopes, the maximum of three
 // negative ones, else zero -- selected through min3/max3.
-__device__ __forceinline__ float minmod3(float a, float b, float c)
+__device__ __forceinline__ real minmod3(real a, real b, real c)
 {
-    const float mn = fminf(fminf(a, b), c);
-    const float mx = fmaxf(fmaxf(a, b), c);
+    const real mn = FMIN(FMIN(a, b), c);
+    const real mx = FMAX(FMAX(a, b), c);
     return mn > 0.f ? mn : (mx < 0.f ? mx : 0.f);
 }
 
 // Reconstruct, compute.hlsl:298-315: half-slope of the generalized minmod limiter
-__device__ __forceinline__ float half_slope(float two_theta, float west, float here, float east)
+__device__ __forceinline__ real half_slope(real two_theta, real west, real here, real east)
 {
-    const float z1 = two_theta * (here - west);
-    const float z2 = (east - west);
-    const float z3 = two_theta * (east - here);
+    const real z1 = two_theta * (here - west);
+    const real z2 = (east - west);
+    const real z3 = two_theta * (east - here);
     return 0.25f * minmod3(z1, z2, z3);
 }
 
 // CorrectW, compute.hlsl:317-328, as selects
-__device__ __forceinline__ void correct_w(float B_west, float B_east, float w_bar, float &w_west, float &w_east)
+__device__ __forceinline__ void correct_w(real B_west, real B_east, real w_bar, real &w_west, real &w_east)
 {
     const bool c1 = w_east < B_east;
     const bool c2 = !c1 && w_west < B_west;
-    const float alt_w = fmaxr(B_west, 2.f * w_bar - B_east);
-    const float alt_e = fmaxr(B_east, 2.f * w_bar - B_west);
-    const float ne = c1 ? B_east : (c2 ? alt_e : w_east);
-    const float nw = c1 ? alt_w : (c2 ? B_west : w_west);
+    const real alt_w = fmaxr(B_west, 2.f * w_bar - B_east);
+    const real alt_e = fmaxr(B_east, 2.f * w_bar - B_west);
+    const real ne = c1 ? B_east : (c2 ? alt_e : w_east);
+    const real nw = c1 ? alt_w : (c2 ? B_west : w_west);
     w_east = ne;
     w_west = nw;
 }
 
 // desingularised 1/h of CalcUV, compute.hlsl:330-340
-__device__ __forceinline__ float divide_by_h(float h, float eps)
+__device__ __forceinline__ real divide_by_h(real h, real eps)
 {
-    const float h2 = h * h;
-    const float h4 = h2 * h2;
-    return 1.41421356237309504880f * h / sqrtf(h4 + fmaxr(h4, eps));
+    const real h2 = h * h;
+    const real h4 = h2 * h2;
+    return SQRT(real(2)) * h / SQRT(h4 + fmaxr(h4, eps));
 }
 
 // NumericalFlux, compute.hlsl:363-370
-__device__ __forceinline__ float num_flux(float ap, float am, float Fp, float Fm, float dU)
+__device__ __forceinline__ real num_flux(real ap, real am, real Fp, real Fm, real dU)
 {
-    const float den = ap - am;
-    const float q = (ap * Fm - am * Fp + ap * am * dU) / den;   // discarded when den == 0
+    const real den = ap - am;
+    const real q = (ap * Fm - am * Fp + ap * am * dU) / den;   // discarded when den == 0
     return den != 0.f ? q : 0.f;
 }
 
 // Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no contraction).
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+__device__ __forceinline__ void two_sum(real a, real b, real &s, real &e)
 {
     s = a + b;
-    const float bb = s - a;
+    const real bb = s - a;
     e = (a - (s - bb)) + (b - bb);
 }
 
 // Element access through a 32-bit byte offset from a uniform base: maps to the
 // scalar-base + vector-offset global addressing mode (planes are < 4 GiB).
-__device__ __forceinline__ float ld32(const float *base, unsigned idx)
+__device__ __forceinline__ real ld32(const real *base, unsigned idx)
 {
-    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (idx << 2));
+    return *reinterpret_cast<const real *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(real));
 }
-__device__ __forceinline__ void st32(float *base, unsigned idx, float v)
+__device__ __forceinline__ void st32(real *base, unsigned idx, real v)
 {
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (idx << 2)) = v;
+    *reinterpret_cast<real *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(real)) = v;
 }
 
-struct Face { float h, u, v; };
+struct Face { real h, u, v; };
 
 // Face values of one cell along one axis (Pass1, compute.hlsl:408-433): returns the
 // "low" face (W or S) and the "high" face (E or N).
-__device__ __forceinline__ void faces_1d(float two_theta, float eps,
-                                         float w_lo, float w_c, float w_hi,
-                                         float hu_lo, float hu_c, float hu_hi,
-                                         float hv_lo, float hv_c, float hv_hi,
-                                         float B_lo, float B_hi, Face &lo, Face &hi)
+__device__ __forceinline__ void faces_1d(real two_theta, real eps,
+                                         real w_lo, real w_c, real w_hi,
+                                         real hu_lo, real hu_c, real hu_hi,
+                                         real hv_lo, real hv_c, real hv_hi,
+                                         real B_lo, real B_hi, Face &lo, Face &hi)
 {
-    const float sw = half_slope(two_theta, w_lo, w_c, w_hi);
-    float w_hi_f = w_c + sw, w_lo_f = w_c - sw;
-    const float su = half_slope(two_theta, hu_lo, hu_c, hu_hi);
-    const float hu_hi_f = hu_c + su, hu_lo_f = hu_c - su;
-    const float sv = half_slope(two_theta, hv_lo, hv_c, hv_hi);
-    const float hv_hi_f = hv_c + sv, hv_lo_f = hv_c - sv;
+    const real sw = half_slope(two_theta, w_lo, w_c, w_hi);
+    real w_hi_f = w_c + sw, w_lo_f = w_c - sw;
+    const real su = half_slope(two_theta, hu_lo, hu_c, hu_hi);
+    const real hu_hi_f = hu_c + su, hu_lo_f = hu_c - su;
+    const real sv = half_slope(two_theta, hv_lo, hv_c, hv_hi);
+    const real hv_hi_f = hv_c + sv, hv_lo_f = hv_c - sv;
     correct_w(B_lo, B_hi, w_c, w_lo_f, w_hi_f);
     hi.h = w_hi_f - B_hi;
     lo.h = w_lo_f - B_lo;
-    const float qh = divide_by_h(hi.h, eps);
+    const real qh = divide_by_h(hi.h, eps);
     hi.u = qh * hu_hi_f;
     hi.v = qh * hv_hi_f;
-    const float ql = divide_by_h(lo.h, eps);
+    const real ql = divide_by_h(lo.h, eps);
     lo.u = ql * hu_lo_f;
     lo.v = ql * hv_lo_f;
 }
@@ -141,24 +170,24 @@ __device__ __forceinline__ void faces_1d(float two_theta, float eps,
 // Pass2 x-direction flux between cell E face (here) and the W face of the east cell.
 // normal velocity u, tangential v; for y-direction call with (v, u) swapped and the
 // momentum components mapped as in compute.hlsl:517-533.
-__device__ __forceinline__ void flux_x(float g, float half_g, const Face &E, const Face &Wn, float &f0, float &f1, float &f2)
+__device__ __forceinline__ void flux_x(real g, real half_g, const Face &E, const Face &Wn, real &f0, real &f1, real &f2)
 {
-    const float cE = sqrtf(g * E.h);
-    const float cW = sqrtf(g * Wn.h);
-    const float ap = fmaxr(fmaxr(E.u + cE, Wn.u + cW), 0.f);
-    const float am = fminr(fminr(E.u - cE, Wn.u - cW), 0.f);
+    const real cE = SQRT(g * E.h);
+    const real cW = SQRT(g * Wn.h);
+    const real ap = fmaxr(fmaxr(E.u + cE, Wn.u + cW), 0.f);
+    const real am = fminr(fminr(E.u - cE, Wn.u - cW), 0.f);
     f0 = num_flux(ap, am, Wn.h * Wn.u, E.h * E.u, Wn.h - E.h);
     f1 = num_flux(ap, am, Wn.h * (Wn.u * Wn.u + half_g * Wn.h), E.h * (E.u * E.u + half_g * E.h),
                   Wn.h * Wn.u - E.h * E.u);
     f2 = num_flux(ap, am, Wn.h * Wn.u * Wn.v, E.h * E.u * E.v, Wn.h * Wn.v - E.h * E.v);
 }
 
-__device__ __forceinline__ void flux_y(float g, float half_g, const Face &N, const Face &Sn, float &f0, float &f1, float &f2)
+__device__ __forceinline__ void flux_y(real g, real half_g, const Face &N, const Face &Sn, real &f0, real &f1, real &f2)
 {
-    const float cN = sqrtf(g * N.h);
-    const float cS = sqrtf(g * Sn.h);
-    const float bp = fmaxr(fmaxr(N.v + cN, Sn.v + cS), 0.f);
-    const float bm = fminr(fminr(N.v - cN, Sn.v - cS), 0.f);
+    const real cN = SQRT(g * N.h);
+    const real cS = SQRT(g * Sn.h);
+    const real bp = fmaxr(fmaxr(N.v + cN, Sn.v + cS), 0.f);
+    const real bm = fminr(fminr(N.v - cN, Sn.v - cS), 0.f);
     f0 = num_flux(bp, bm, Sn.h * Sn.v, N.h * N.v, Sn.h - N.h);
     f1 = num_flux(bp, bm, Sn.h * Sn.u * Sn.v, N.h * N.u * N.v, Sn.h * Sn.u - N.h * N.u);
     f2 = num_flux(bp, bm, Sn.h * (Sn.v * Sn.v + half_g * Sn.h), N.h * (N.v * N.v + half_g * N.h),
@@ -167,14 +196,14 @@ __device__ __forceinline__ void flux_y(float g, float half_g, const Face &N, con
 
 // Level-0 tridiagonal coefficients (engine.cpp:2664-2680) of a line cell: K1 and K2/72
 // from the depths at offsets -2..+2 along the line, then A, B, C.
-__device__ __forceinline__ void line_kq(float kcoef, float d2, float dm2, float dm1, float d0, float dp1, float dp2,
-                                        float &K1, float &Q)
+__device__ __forceinline__ void line_kq(real kcoef, real d2, real dm2, real dm1, real d0, real dp1, real dp2,
+                                        real &K1, real &Q)
 {
     K1 = kcoef * (d0 * d0 / d2);
-    const float K2 = (d0 / d2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
+    const real K2 = (d0 / d2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
     Q = K2 / 72.f;
 }
-__device__ __forceinline__ void line_abc(float K1, float Q, bool first, bool last, float &A, float &B, float &C)
+__device__ __forceinline__ void line_abc(real K1, real Q, bool first, bool last, real &A, real &B, real &C)
 {
     A = first ? 0.f : -K1 + Q;
     B = 1.f + 2.f * K1;
@@ -185,9 +214,9 @@ __device__ __forceinline__ void line_abc(float K1, float Q, bool first, bool las
 
 // Per-cell inputs of Pass3 that are read once: Adams-Bashforth history and static fields.
 struct CellIn {
-    float o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
-    float Kx, Qx, Ky, Qy, ddx, ddy;
-    float wlo;      // low word of w (compensated state)
+    real o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
+    real Kx, Qx, Ky, Qy, ddx, ddy;
+    real wlo;      // low word of w (compensated state)
 };
 
 template <int SCHEME, bool COMP>
@@ -213,9 +242,9 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
 // and the y-line right-hand sides, stored transposed after the band.
 template <bool CORRECTOR, int R>
 struct ExplicitSmem {
-    float hv[R + 4][64];
-    float chv[CORRECTOR ? R + 4 : 1][64];
-    float dy[R][64];
+    real hv[R + 4][64];
+    real chv[CORRECTOR ? R + 4 : 1][64];
+    real dy[R][64];
 };
 
 // One wave computes interior columns [x0, x0 + 60) of rows [ys, ys + R) of one scenario.
@@ -223,9 +252,9 @@ struct ExplicitSmem {
 template <int SCHEME, bool COMP, int R, bool CS>
 __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS, R> &sm)
 {
-    float (*hvblk)[64] = sm.hv;
-    float (*chvblk)[64] = sm.chv;
-    float (*dystg)[64] = sm.dy;
+    real (*hvblk)[64] = sm.hv;
+    real (*chvblk)[64] = sm.chv;
+    real (*dystg)[64] = sm.dy;
     const int lane = threadIdx.x;
     const int strip = tile % p.nstrips;
     const int band = tile / p.nstrips;
@@ -239,39 +268,37 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     const int P = p.P, PT = p.PT;
 
     // transposed hv: each lane stages rows [ys-2, ye+2) of its own column (16-byte loads;
-    // ys-2 is a multiple of 4 and the column pitch leaves >= 16 floats of padding)
+    // ys-2 is a multiple of 4 and the column pitch leaves >= 16 elements of padding)
     {
         const int nrows = ye - ys + 4;
-        const float4 *src = reinterpret_cast<const float4 *>(p.hvT + (size_t)xc * PT + (ys - 2));
-        for (int r = 0; r < nrows; r += 4) {
-            const float4 v = src[r >> 2];
-            hvblk[r][lane] = v.x;
-            if (r + 1 < R + 4) hvblk[r + 1][lane] = v.y;
-            if (r + 2 < R + 4) hvblk[r + 2][lane] = v.z;
-            if (r + 3 < R + 4) hvblk[r + 3][lane] = v.w;
+        const Vec *src = reinterpret_cast<const Vec *>(p.hvT + (size_t)xc * PT + (ys - 2));
+        for (int r = 0; r < nrows; r += kVec) {
+            const Vec v = src[r / kVec];
+#pragma unroll
+            for (int j = 0; j < kVec; ++j)
+                if (r + j < R + 4) hvblk[r + j][lane] = v.v[j];
         }
         if (SCHEME == 2) {
-            const float4 *csrc = reinterpret_cast<const float4 *>(p.ChvT + (size_t)xc * PT + (ys - 2));
-            for (int r = 0; r < nrows; r += 4) {
-                const float4 v = csrc[r >> 2];
-                chvblk[r][lane] = v.x;
-                if (r + 1 < R + 4) chvblk[r + 1][lane] = v.y;
-                if (r + 2 < R + 4) chvblk[r + 2][lane] = v.z;
-                if (r + 3 < R + 4) chvblk[r + 3][lane] = v.w;
+            const Vec *csrc = reinterpret_cast<const Vec *>(p.ChvT + (size_t)xc * PT + (ys - 2));
+            for (int r = 0; r < nrows; r += kVec) {
+                const Vec v = csrc[r / kVec];
+#pragma unroll
+                for (int j = 0; j < kVec; ++j)
+                    if (r + j < R + 4) chvblk[r + j][lane] = v.v[j];
             }
         }
     }
     __syncthreads();
 
-    const float tt = p.two_theta, eps = p.epsilon, g = p.g, half_g = p.half_g;
-    const float sl = p.sea_level;
+    const real tt = p.two_theta, eps = p.epsilon, g = p.g, half_g = p.half_g;
+    const real sl = p.sea_level;
 
     // register windows over rows; index 0 = y-2 ... 4 = y+2
-    float w_[5], hu_[4], hv_[4], BY_[4];
+    real w_[5], hu_[4], hv_[4], BY_[4];
     // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
     // 32-bit element offsets: planes are far below 4 G elements, and a uniform base with a
     // 32-bit offset maps to the scalar-base addressing mode (no 64-bit address arithmetic)
-    auto ldrow = [&](const float *base, int row) { return ld32(base, (unsigned)row * (unsigned)P + (unsigned)xc); };
+    auto ldrow = [&](const real *base, int row) { return ld32(base, (unsigned)row * (unsigned)P + (unsigned)xc); };
 
     // prologue: rows ys-2 .. ys+1
     for (int j = 0; j < 4; ++j) {
@@ -284,9 +311,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         hv_[j] = hvblk[row - (ys - 2)][lane];
         BY_[j] = ldrow(p.BY, row);
     }
-    const float hu_m2 = ldrow(p.hu, ys - 2);
-    const float hv_m2 = hvblk[0][lane];
-    const float BY_m2 = ldrow(p.BY, ys - 2);
+    const real hu_m2 = ldrow(p.hu, ys - 2);
+    const real hv_m2 = hvblk[0][lane];
+    const real BY_m2 = ldrow(p.BY, ys - 2);
 
     // faces of row ys-1 (need rows ys-2..ys) -> keep N; faces of row ys -> S, N
     Face Nprev, Scur, Ncur;
@@ -298,13 +325,13 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         Scur = lo;
         Ncur = hi;
     }
-    float yf_s0, yf_s1, yf_s2;   // yflux at row y-1
+    real yf_s0, yf_s1, yf_s2;   // yflux at row y-1
     flux_y(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
 
     // software pipelining: the loads a row needs are issued one row ahead
     const int ylast = p.ny + 3;
-    float nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
-    float nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys);
+    real nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
+    real nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys);
     CellIn nx_c = load_cell<SCHEME, COMP>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
 
     for (int y = ys; y < ye; ++y) {
@@ -313,8 +340,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         hu_[3] = nx_hu;
         hv_[3] = hvblk[y + 2 - (ys - 2)][lane];
         BY_[3] = nx_BY;
-        const float BXc = nx_BX;
-        const float BAc = nx_BA;
+        const real BXc = nx_BX;
+        const real BAc = nx_BA;
         const CellIn cin = nx_c;
         {
             const int yn = min(y + 3, ylast), yc = min(y + 1, ye - 1);
@@ -329,15 +356,15 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         // ---- faces of row y+1 along y: S face used now, N face kept
         Face S1, N1;
         faces_1d(tt, eps, w_[2], w_[3], w_[4], hu_[1], hu_[2], hu_[3], hv_[1], hv_[2], hv_[3], BY_[1], BY_[2], S1, N1);
-        float yf0, yf1, yf2;   // yflux at row y
+        real yf0, yf1, yf2;   // yflux at row y
         flux_y(g, half_g, Ncur, S1, yf0, yf1, yf2);
 
         // ---- faces of row y along x
-        const float wc = w_[2], huc = hu_[1], hvc = hv_[1];
-        const float wl = from_left(wc), wr = from_right(wc);
-        const float hul = from_left(huc), hur = from_right(huc);
-        const float hvl = from_left(hvc), hvr = from_right(hvc);
-        const float BXw = from_left(BXc);
+        const real wc = w_[2], huc = hu_[1], hvc = hv_[1];
+        const real wl = from_left(wc), wr = from_right(wc);
+        const real hul = from_left(huc), hur = from_right(huc);
+        const real hvl = from_left(hvc), hvr = from_right(hvc);
+        const real BXw = from_left(BXc);
         Face Wf, Ef;
         faces_1d(tt, eps, wl, wc, wr, hul, huc, hur, hvl, hvc, hvr, BXw, BXc, Wf, Ef);
         // Pass1 stores u = (1/h)*hu and v = (1/h)*hv; along x the normal velocity is u.
@@ -345,89 +372,89 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         Wn.h = from_right(Wf.h);
         Wn.u = from_right(Wf.u);
         Wn.v = from_right(Wf.v);
-        float xf0, xf1, xf2;   // xflux at (x, y)
+        real xf0, xf1, xf2;   // xflux at (x, y)
         flux_x(g, half_g, Ef, Wn, xf0, xf1, xf2);
-        const float xw0 = from_left(xf0), xw1 = from_left(xf1), xw2 = from_left(xf2);
+        const real xw0 = from_left(xf0), xw1 = from_left(xf1), xw2 = from_left(xf2);
 
         // ---- Pass3 at (x, y), compute.hlsl:616-812
-        const float wll = from_left(wl), wrr = from_right(wr);
-        const float wul = from_left(w_[3]), wur = from_right(w_[3]);
-        const float wdl = from_left(w_[1]), wdr = from_right(w_[1]);
-        const float BYs = BY_[0];
-        const float BYc = BY_[1];
+        const real wll = from_left(wl), wrr = from_right(wr);
+        const real wul = from_left(w_[3]), wur = from_right(w_[3]);
+        const real wdl = from_left(w_[1]), wdr = from_right(w_[1]);
+        const real BYs = BY_[0];
+        const real BYc = BY_[1];
 
         if (out_lane) {
             const unsigned i = (unsigned)y * (unsigned)P + (unsigned)x;
-            const float h = fmaxr(0.f, wc - BAc);
-            const float d_here = fmaxr(0.f, sl - BAc);
-            const float d2_here = d_here * d_here;
-            const float d3_here = d2_here * d_here;
+            const real h = fmaxr(0.f, wc - BAc);
+            const real d_here = fmaxr(0.f, sl - BAc);
+            const real d2_here = d_here * d_here;
+            const real d3_here = d2_here * d_here;
 
-            const float eta_here = wc - sl;
-            const float eta_left = wl - sl, eta_right = wr - sl;
-            const float eta_down = w_[1] - sl, eta_up = w_[3] - sl;
-            const float eta_left_left = wll - sl, eta_right_right = wrr - sl;
-            const float eta_down_down = w_[0] - sl, eta_up_up = w_[4] - sl;
-            const float eta_up_left = wul - sl, eta_up_right = wur - sl;
-            const float eta_down_left = wdl - sl, eta_down_right = wdr - sl;
+            const real eta_here = wc - sl;
+            const real eta_left = wl - sl, eta_right = wr - sl;
+            const real eta_down = w_[1] - sl, eta_up = w_[3] - sl;
+            const real eta_left_left = wll - sl, eta_right_right = wrr - sl;
+            const real eta_down_down = w_[0] - sl, eta_up_up = w_[4] - sl;
+            const real eta_up_left = wul - sl, eta_up_right = wur - sl;
+            const real eta_down_left = wdl - sl, eta_down_right = wdr - sl;
 
-            const float odx = p.one_over_dx, ody = p.one_over_dy;
-            const float one_over_d2x = odx * odx;
-            const float one_over_d3x = one_over_d2x * odx;
-            const float one_over_d2y = ody * ody;
-            const float one_over_d3y = one_over_d2y * ody;
+            const real odx = p.one_over_dx, ody = p.one_over_dy;
+            const real one_over_d2x = odx * odx;
+            const real one_over_d3x = one_over_d2x * odx;
+            const real one_over_d2y = ody * ody;
+            const real one_over_d3y = one_over_d2y * ody;
 
-            const float dd_by_dx = cin.ddx;
-            const float dd_by_dy = cin.ddy;
-            const float eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
-            const float eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
-            const float eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
+            const real dd_by_dx = cin.ddx;
+            const real dd_by_dy = cin.ddy;
+            const real eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
+            const real eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
+            const real eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
 
-            const float Bg = p.Bcoef_g;
-            const float Psi1x = Bg * (d3_here) *
+            const real Bg = p.Bcoef_g;
+            const real Psi1x = Bg * (d3_here) *
                 ((eta_right_right - 2.f * eta_right + 2.f * eta_left - eta_left_left) * (0.5f * one_over_d3x)
                  + (eta_up_right - eta_up_left - 2.f * eta_right + 2.f * eta_left + eta_down_right - eta_down_left)
                    * (0.5f * odx * one_over_d2y));
-            const float Psi2x = Bg * (d2_here) * (dd_by_dx * (2.f * eta_by_dx_dx + eta_by_dy_dy) + dd_by_dy * eta_by_dx_dy);
-            const float Psi1y = Bg * (d3_here) *
+            const real Psi2x = Bg * (d2_here) * (dd_by_dx * (2.f * eta_by_dx_dx + eta_by_dy_dy) + dd_by_dy * eta_by_dx_dy);
+            const real Psi1y = Bg * (d3_here) *
                 ((eta_up_up - 2.f * eta_up + 2.f * eta_down - eta_down_down) * (0.5f * one_over_d3y)
                  + (eta_up_right + eta_up_left - 2.f * eta_up + 2.f * eta_down - eta_down_right - eta_down_left)
                    * (0.5f * ody * one_over_d2x));
-            const float Psi2y = Bg * (d2_here) * (dd_by_dy * (eta_by_dx_dx + 2.f * eta_by_dy_dy) + dd_by_dx * eta_by_dx_dy);
+            const real Psi2y = Bg * (d2_here) * (dd_by_dy * (eta_by_dx_dx + 2.f * eta_by_dy_dy) + dd_by_dx * eta_by_dx_dy);
 
             // FrictionCalc, compute.hlsl:544-553 (pow(h,4) as (h*h)*(h*h))
-            float friction_ = 0.f;
+            real friction_ = 0.f;
             if (p.friction != 0.f) {
-                float f = p.friction;
-                const float dbh = divide_by_h(h, eps);
-                if (p.is_manning == 1) f = g * p.friction * p.friction * powf(fabsf(dbh), 0.3333f);
-                friction_ = f * sqrtf(huc * huc + hvc * hvc) * dbh * dbh;
+                real f = p.friction;
+                const real dbh = divide_by_h(h, eps);
+                if (p.is_manning == 1) f = g * p.friction * p.friction * POW(FABS(dbh), (real)0.3333f);
+                friction_ = f * SQRT(huc * huc + hvc * hvc) * dbh * dbh;
             }
-            const float src_g = -p.g_over_dx * h * (BXc - BXw) - huc * friction_ + (Psi1x + Psi2x);
-            const float src_b = -p.g_over_dy * h * (BYc - BYs) - hvc * friction_ + (Psi1y + Psi2y);
+            const real src_g = -p.g_over_dx * h * (BXc - BXw) - huc * friction_ + (Psi1x + Psi2x);
+            const real src_b = -p.g_over_dy * h * (BYc - BYs) - hvc * friction_ + (Psi1y + Psi2y);
 
-            const float dr = (xw0 - xf0) * odx + (yf_s0 - yf0) * ody + 0.f;
-            const float dg = (xw1 - xf1) * odx + (yf_s1 - yf1) * ody + src_g;
-            const float db = (xw2 - xf2) * odx + (yf_s2 - yf2) * ody + src_b;
+            const real dr = (xw0 - xf0) * odx + (yf_s0 - yf0) * ody + 0.f;
+            const real dg = (xw1 - xf1) * odx + (yf_s1 - yf1) * ody + src_g;
+            const real db = (xw2 - xf2) * odx + (yf_s2 - yf2) * ody + src_b;
 
             // F*, G* (compute.hlsl:726-734); the "0 *" cross term of the reference adds zero
-            const float hv_up = hv_[2], hv_down = hv_[0];
-            const float hu_up = hu_[2], hu_down = hu_[0];
-            const float F_star = (1.f / 6.f * d_here *
+            const real hv_up = hv_[2], hv_down = hv_[0];
+            const real hu_up = hu_[2], hu_down = hu_[0];
+            const real F_star = (real(1) / real(6) * d_here *
                 (dd_by_dx * ((ody / 2.f) * (hv_up - hv_down)) + (dd_by_dy * ((odx / 2.f) * (hvr - hvl))))) + 0.f;
-            const float G_star = (1.f / 6.f * d_here *
+            const real G_star = (real(1) / real(6) * d_here *
                 (dd_by_dx * ((ody / 2.f) * (hu_up - hu_down)) + (dd_by_dy * ((odx / 2.f) * (hur - hul))))) + 0.f;
 
             // level-0 coefficients of the x-line (row y) and y-line (column x) through this cell
-            float Ax, Bx, Cx, Ay, By, Cy;
+            real Ax, Bx, Cx, Ay, By, Cy;
             line_abc(cin.Kx, cin.Qx, x == 2, x == p.nx + 1, Ax, Bx, Cx);
             line_abc(cin.Ky, cin.Qy, y == 2, y == p.ny + 1, Ay, By, Cy);
 
-            const float dt = p.dt;
-            float wOut, huOut, hvOut;
+            const real dt = p.dt;
+            real wOut, huOut, hvOut;
             // increments of w and the line right-hand sides without the level-0 A, B, C
             // terms: the compensated update solves for the increment of hu and hv
-            float dw, dhu, dhv;
+            real dw, dhu, dhv;
             if (SCHEME == 0) {
                 dw = dt * dr;
                 dhu = dt * dg;
@@ -436,44 +463,44 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                 huOut = Ax * hul + Bx * huc + Cx * hur + dhu;
                 hvOut = Ay * hv_down + By * hvc + Cy * hv_up + dhv;
             } else {
-                const float o0 = cin.o0, o1 = cin.o1, o2 = cin.o2;
-                const float oo0 = cin.oo0, oo1 = cin.oo1, oo2 = cin.oo2;
-                const float Fo = cin.Fo, Go_ = cin.Go;
-                const float Foo = cin.Foo, Goo_ = cin.Goo;
+                const real o0 = cin.o0, o1 = cin.o1, o2 = cin.o2;
+                const real oo0 = cin.oo0, oo1 = cin.oo1, oo2 = cin.oo2;
+                const real Fo = cin.Fo, Go_ = cin.Go;
+                const real Foo = cin.Foo, Goo_ = cin.Goo;
                 if (SCHEME == 1) {
                     dw = dt / 12.f * (23.f * dr - 16.f * o0 + 5.f * oo0);
-                    const float u1 = dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1), u2 = 2.f * F_star - 3.f * Fo + Foo;
-                    const float v1 = dt / 12.f * (23.f * db - 16.f * o2 + 5.f * oo2), v2 = 2.f * G_star - 3.f * Go_ + Goo_;
+                    const real u1 = dt / 12.f * (23.f * dg - 16.f * o1 + 5.f * oo1), u2 = 2.f * F_star - 3.f * Fo + Foo;
+                    const real v1 = dt / 12.f * (23.f * db - 16.f * o2 + 5.f * oo2), v2 = 2.f * G_star - 3.f * Go_ + Goo_;
                     wOut = wc + dw;
                     huOut = Ax * hul + Bx * huc + Cx * hur + u1 + u2;
                     hvOut = Ay * hv_down + By * hvc + Cy * hv_up + v1 + v2;
                     dhu = u1 + u2;
                     dhv = v1 + v2;
                 } else if (SCHEME == 3) {
-                    const float *c = p.der;
-                    const float dF0 = (c[0] * F_star) + (c[1] * Fo) + (c[2] * Foo);
-                    const float dF1 = (c[3] * F_star) + (c[4] * Fo) + (c[5] * Foo);
-                    const float dF2 = (c[6] * F_star) + (c[7] * Fo) + (c[8] * Foo);
-                    const float dG0 = (c[0] * G_star) + (c[1] * Go_) + (c[2] * Goo_);
-                    const float dG1 = (c[3] * G_star) + (c[4] * Go_) + (c[5] * Goo_);
-                    const float dG2 = (c[6] * G_star) + (c[7] * Go_) + (c[8] * Goo_);
-                    const float a0 = p.ab[0], a1 = p.ab[1], a2 = p.ab[2];
+                    const real *c = p.der;
+                    const real dF0 = (c[0] * F_star) + (c[1] * Fo) + (c[2] * Foo);
+                    const real dF1 = (c[3] * F_star) + (c[4] * Fo) + (c[5] * Foo);
+                    const real dF2 = (c[6] * F_star) + (c[7] * Fo) + (c[8] * Foo);
+                    const real dG0 = (c[0] * G_star) + (c[1] * Go_) + (c[2] * Goo_);
+                    const real dG1 = (c[3] * G_star) + (c[4] * Go_) + (c[5] * Goo_);
+                    const real dG2 = (c[6] * G_star) + (c[7] * Go_) + (c[8] * Goo_);
+                    const real a0 = p.ab[0], a1 = p.ab[1], a2 = p.ab[2];
                     dw = dt / 6.f * (a0 * dr + a1 * o0 + a2 * oo0);
-                    const float u1 = dt / 6.f * (a0 * dg + a1 * o1 + a2 * oo1), u2 = dt / 6.f * (a0 * dF0 + a1 * dF1 + a2 * dF2);
-                    const float v1 = dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2), v2 = dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
+                    const real u1 = dt / 6.f * (a0 * dg + a1 * o1 + a2 * oo1), u2 = dt / 6.f * (a0 * dF0 + a1 * dF1 + a2 * dF2);
+                    const real v1 = dt / 6.f * (a0 * db + a1 * o2 + a2 * oo2), v2 = dt / 6.f * (a0 * dG0 + a1 * dG1 + a2 * dG2);
                     wOut = wc + dw;
                     huOut = Ax * hul + Bx * huc + Cx * hur + u1 + u2;
                     hvOut = Ay * hv_down + By * hvc + Cy * hv_up + v1 + v2;
                     dhu = u1 + u2;
                     dhv = v1 + v2;
                 } else {
-                    const float cw = ld32(p.Cw, i);
-                    const float chu = ld32(p.Chu, i), chul = ld32(p.Chu, i - 1u), chur = ld32(p.Chu, i + 1u);
-                    const float chv = chvblk[y - (ys - 2)][lane];
-                    const float chvd = chvblk[y - 1 - (ys - 2)][lane];
-                    const float chvu = chvblk[y + 1 - (ys - 2)][lane];
-                    const float pr0 = ld32(p.Gp[0], i), pr1 = ld32(p.Gp[1], i), pr2 = ld32(p.Gp[2], i);
-                    const float Fp = ld32(p.FGp[0], i), Gp_ = ld32(p.FGp[1], i);
+                    const real cw = ld32(p.Cw, i);
+                    const real chu = ld32(p.Chu, i), chul = ld32(p.Chu, i - 1u), chur = ld32(p.Chu, i + 1u);
+                    const real chv = chvblk[y - (ys - 2)][lane];
+                    const real chvd = chvblk[y - 1 - (ys - 2)][lane];
+                    const real chvu = chvblk[y + 1 - (ys - 2)][lane];
+                    const real pr0 = ld32(p.Gp[0], i), pr1 = ld32(p.Gp[1], i), pr2 = ld32(p.Gp[2], i);
+                    const real Fp = ld32(p.FGp[0], i), Gp_ = ld32(p.FGp[1], i);
                     wOut = cw + dt / 24.f * (9.f * dr + 19.f * pr0 - 5.f * o0 + 1.f * oo0);
                     huOut = Ax * chul + Bx * chu + Cx * chur + dt / 24.f * (9.f * dg + 19.f * pr1 - 5.f * o1 + 1.f * oo1) +
                             F_star - Fp;
@@ -483,8 +510,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                 }
             }
             if (COMP) {
-                // (w_hi, w_lo) += dw as a two-float sum; the lines solve for the increments
-                float lo;
+                // (w_hi, w_lo) += dw as a two-real sum; the lines solve for the increments
+                real lo;
                 two_sum(wc, dw + cin.wlo, wOut, lo);
                 st32(p.w_lo_new, i, lo);
                 huOut = dhu;
@@ -509,12 +536,16 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
     // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
     if (out_lane) {
-        float *dst = p.DyT + (size_t)x * PT + (ys - 2);
+        real *dst = p.DyT + (size_t)x * PT + (ys - 2);
         const int nr = ye - ys;
         if (nr == R) {
-            float4 *d4 = reinterpret_cast<float4 *>(dst);
-            for (int r = 0; r < R; r += 4)
-                d4[r >> 2] = make_float4(dystg[r][lane], dystg[r + 1][lane], dystg[r + 2][lane], dystg[r + 3][lane]);
+            Vec *dv = reinterpret_cast<Vec *>(dst);
+            for (int r = 0; r < R; r += kVec) {
+                Vec v;
+#pragma unroll
+                for (int j = 0; j < kVec; ++j) v.v[j] = dystg[r + j][lane];
+                dv[r / kVec] = v;
+            }
         } else {
             for (int r = 0; r < nr; ++r) dst[r] = dystg[r][lane];
         }
@@ -539,27 +570,27 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 
 // Irregular waves: one wave per (side, strip index, line cell) computes that cell's
 // pairwise-tree sum of the component texels (ColumnSumReduce/RowSumReduce order).
-__device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip, int line, float t)
+__device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip, int line, real t)
 {
     const int lane = threadIdx.x & 63;
     const int cell = strip * S.nlines + line;
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    real r0 = 0.f, r1 = 0.f, r2 = 0.f;
     const bool ghost = line < 2 || line >= S.nlines - 2 || S.zero_line;
     if (!ghost) {
         const bool wet = S.wet[cell] != 0;
         const int A = S.nleaf_pow2;                 // power of two, <= 1024
         const int lanes_used = A > 64 ? 64 : A;
         const int nblk = A > 64 ? A / 64 : 1;       // blocks of 64 leaves
-        float b0 = 0.f, b1 = 0.f, b2 = 0.f;         // lane j keeps the sum of block j
+        real b0 = 0.f, b1 = 0.f, b2 = 0.f;         // lane j keeps the sum of block j
         for (int blk = 0; blk < nblk; ++blk) {
             const int wv = blk * 64 + lane;
-            float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+            real v0 = 0.f, v1 = 0.f, v2 = 0.f;
             if (lane < lanes_used) {
-                float e = 0.f, hu = 0.f, hv = 0.f;
+                real e = 0.f, hu = 0.f, hv = 0.f;
                 if (wet && wv < S.nleaf_active) {
-                    const float *tb = S.tab + ((size_t)cell * S.nw + wv) * 4;
-                    const float arg = S.omega[wv] * t - tb[0] - tb[1] + S.phase[wv];
-                    e = S.amp[wv] * sinf(arg);
+                    const real *tb = S.tab + ((size_t)cell * S.nw + wv) * 4;
+                    const real arg = S.omega[wv] * t - tb[0] - tb[1] + S.phase[wv];
+                    e = S.amp[wv] * SIN(arg);
                     hu = tb[2] * e;
                     hv = tb[3] * e;
                 }
@@ -567,7 +598,7 @@ __device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip
             }
             // pairwise tree over the block's leaves (ColumnSumReduce order)
             for (int off = 1; off < lanes_used; off <<= 1) {
-                const float o0 = __shfl_down(v0, off, 64), o1 = __shfl_down(v1, off, 64), o2 = __shfl_down(v2, off, 64);
+                const real o0 = __shfl_down(v0, off, 64), o1 = __shfl_down(v1, off, 64), o2 = __shfl_down(v2, off, 64);
                 if ((lane & (2 * off - 1)) == 0) { v0 = v0 + o0; v1 = v1 + o1; v2 = v2 + o2; }
             }
             v0 = __shfl(v0, 0, 64); v1 = __shfl(v1, 0, 64); v2 = __shfl(v2, 0, 64);
@@ -575,11 +606,11 @@ __device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip
         }
         // upper levels of the tree over the block sums
         for (int off = 1; off < nblk; off <<= 1) {
-            const float o0 = __shfl_down(b0, off, 64), o1 = __shfl_down(b1, off, 64), o2 = __shfl_down(b2, off, 64);
+            const real o0 = __shfl_down(b0, off, 64), o1 = __shfl_down(b1, off, 64), o2 = __shfl_down(b2, off, 64);
             if ((lane & (2 * off - 1)) == 0) { b0 = b0 + o0; b1 = b1 + o1; b2 = b2 + o2; }
         }
         // subtrees beyond the active block hold only inactive texels (sl, 0, 0)
-        float c0 = S.sl, c1 = 0.f, c2 = 0.f;
+        real c0 = S.sl, c1 = 0.f, c2 = 0.f;
         for (int w = 1; w < A; w <<= 1) { c0 = c0 + c0; c1 = c1 + c1; c2 = c2 + c2; }
         for (int l = 0; l < S.levels_above; ++l) {
             b0 = b0 + c0; b1 = b1 + c1; b2 = b2 + c2;
@@ -603,7 +634,7 @@ __device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip
 // 2^k <= C they lie inside one lane's chunk, above that at the chunk's last slot of
 // lanes 2^(k-log2 C) (m+1) - 1.  Neighbours outside the chunk come from other lanes
 // through cross-lane reads; positions >= n are the reference's (0, 1, 0 | 0) margin.
-struct Eq { float a, b, c, d; };
+struct Eq { real a, b, c, d; };
 
 __device__ __forceinline__ Eq shfl_eq(const Eq &e, int src)
 {
@@ -612,8 +643,8 @@ __device__ __forceinline__ Eq shfl_eq(const Eq &e, int src)
 
 __device__ __forceinline__ Eq reduce_eq(const Eq &L, const Eq &H, const Eq &R)
 {
-    const float k1 = H.a / L.b;
-    const float k2 = H.c / R.b;
+    const real k1 = H.a / L.b;
+    const real k2 = H.c / R.b;
     Eq o;
     o.a = -L.a * k1;
     o.b = H.b - L.c * k1 - R.a * k2;
@@ -626,21 +657,23 @@ template <int C, int W>
 struct LineSmem {
     // one unknown per thread above the chunk levels, as four planes: level j touches every
     // 2^j-th thread, which a 16-byte record per thread would map onto a quarter of the banks
-    float ta[64 * W], tb[64 * W], tc[64 * W], td[64 * W];
+    real ta[64 * W], tb[64 * W], tc[64 * W], td[64 * W];
     Eq xch[2][64 * W];                         // chunk-level neighbour exchange (double-buffered)
     __device__ Eq top(int i) const { return Eq{ta[i], tb[i], tc[i], td[i]}; }
     __device__ void set_top(int i, const Eq &e) { ta[i] = e.a; tb[i] = e.b; tc[i] = e.c; td[i] = e.d; }
 };
 
-// Loads C consecutive floats starting at a 16-byte aligned element (C % 4 == 0) or singly.
+// Loads C consecutive elements starting at a 16-byte aligned element (C a multiple of
+// kVec) or singly.
 template <int C>
-__device__ __forceinline__ void load_chunk(const float *src, int base, int n, float *v)
+__device__ __forceinline__ void load_chunk(const real *src, int base, int n, real *v)
 {
-    if (C % 4 == 0 && base + C <= n) {
+    if (C % kVec == 0 && base + C <= n) {
 #pragma unroll
-        for (int i = 0; i < C; i += 4) {
-            const float4 f = *reinterpret_cast<const float4 *>(src + base + i);
-            v[i] = f.x; v[i + 1] = f.y; v[i + 2] = f.z; v[i + 3] = f.w;
+        for (int i = 0; i < C; i += kVec) {
+            const Vec f = *reinterpret_cast<const Vec *>(src + base + i);
+#pragma unroll
+            for (int j = 0; j < kVec; ++j) v[i + j] = f.v[j];
         }
     } else {
 #pragma unroll
@@ -651,12 +684,12 @@ __device__ __forceinline__ void load_chunk(const float *src, int base, int n, fl
 // Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
 // D, K, Q point at element 0 of the line (16-byte aligned); out at element 0 of the
 // ghost-inclusive output line + 2 (8-byte aligned).
-// comp: the solution is an increment; out/out_lo receive the two-float sum of the previous
+// comp: the solution is an increment; out/out_lo receive the two-real sum of the previous
 // state (prev, prev_lo, same layout as out) and the increment.
 template <int C, int W>
-__device__ __forceinline__ void cr_line_block(int n, const float *D, const float *K, const float *Qf, float *out,
-                                              LineSmem<C, W> &sm, bool comp, const float *prev, const float *prev_lo,
-                                              float *out_lo)
+__device__ __forceinline__ void cr_line_block(int n, const real *D, const real *K, const real *Qf, real *out,
+                                              LineSmem<C, W> &sm, bool comp, const real *prev, const real *prev_lo,
+                                              real *out_lo)
 {
     constexpr int NT = 64 * W;
     constexpr int K0 = C == 1 ? 0 : C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : C == 16 ? 4 : 5;
@@ -666,7 +699,7 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
     const int base = t * C;
     Eq q[C];
     {
-        float dv[C], kv[C], qv[C];
+        real dv[C], kv[C], qv[C];
         load_chunk<C>(D, base, n, dv);
         load_chunk<C>(K, base, n, kv);
         load_chunk<C>(Qf, base, n, qv);
@@ -719,15 +752,15 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
     for (int j = KT; j >= 0; --j) {
         const int s = 1 << j;
         if (((t + 1) & ((2 << j) - 1)) == s && mypos < n) {
-            const float xl = t - s >= 0 ? sm.td[t - s] : 0.f;
-            const float xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.td[t + s] : 0.f;
+            const real xl = t - s >= 0 ? sm.td[t - s] : 0.f;
+            const real xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.td[t + s] : 0.f;
             const Eq e = sm.top(t);
             sm.td[t] = (e.d - e.a * xl - e.c * xr) / e.b;
         }
         __syncthreads();
     }
     q[C - 1].d = sm.td[t];
-    const float prev_last = t > 0 ? sm.td[t - 1] : 0.f;
+    const real prev_last = t > 0 ? sm.td[t - 1] : 0.f;
     // ---- back-substitution inside the chunk: levels K0-1 .. 0
 #pragma unroll
     for (int k = K0 - 1; k >= 0; --k) {
@@ -736,34 +769,34 @@ __device__ __forceinline__ void cr_line_block(int n, const float *D, const float
         for (int qq = 0; qq < (C >> (k + 1)); ++qq) {
             const int pp = s * (2 * qq + 1) - 1;
             if (base + pp >= n) continue;
-            const float xl = (pp - s >= 0) ? q[pp - s].d : prev_last;
-            const float xr = (base + pp + s < n) ? q[pp + s].d : 0.f;
+            const real xl = (pp - s >= 0) ? q[pp - s].d : prev_last;
+            const real xr = (base + pp + s < n) ? q[pp + s].d : 0.f;
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
     if (comp && C % 2 == 0 && base + C <= n) {
 #pragma unroll
         for (int i = 0; i < C; i += 2) {
-            const float2 pv = *reinterpret_cast<const float2 *>(prev + base + i);
-            const float2 pl = *reinterpret_cast<const float2 *>(prev_lo + base + i);
-            float h0, l0, h1, l1;
+            const Pair pv = *reinterpret_cast<const Pair *>(prev + base + i);
+            const Pair pl = *reinterpret_cast<const Pair *>(prev_lo + base + i);
+            real h0, l0, h1, l1;
             two_sum(pv.x, q[i].d + pl.x, h0, l0);
             two_sum(pv.y, q[i + 1].d + pl.y, h1, l1);
-            *reinterpret_cast<float2 *>(out + base + i) = make_float2(h0, h1);
-            *reinterpret_cast<float2 *>(out_lo + base + i) = make_float2(l0, l1);
+            *reinterpret_cast<Pair *>(out + base + i) = Pair{h0, h1};
+            *reinterpret_cast<Pair *>(out_lo + base + i) = Pair{l0, l1};
         }
     } else if (comp) {
 #pragma unroll
         for (int i = 0; i < C; ++i)
             if (base + i < n) {
-                float hi, lo;
+                real hi, lo;
                 two_sum(prev[base + i], q[i].d + prev_lo[base + i], hi, lo);
                 out[base + i] = hi;
                 out_lo[base + i] = lo;
             }
     } else if (C % 2 == 0 && base + C <= n) {
 #pragma unroll
-        for (int i = 0; i < C; i += 2) *reinterpret_cast<float2 *>(out + base + i) = make_float2(q[i].d, q[i + 1].d);
+        for (int i = 0; i < C; i += 2) *reinterpret_cast<Pair *>(out + base + i) = Pair{q[i].d, q[i + 1].d};
     } else {
 #pragma unroll
         for (int i = 0; i < C; ++i)
@@ -782,7 +815,7 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     const int nlines = p.nlines_x + p.nlines_y;
     if (bid >= nlines) {
         // irregular-wave lines: W per block (one per wave); time t^n from the step clock
-        const float t = irr.clock ? irr.clock[0] : irr.t;
+        const real t = irr.clock ? irr.clock[0] : irr.t;
         int idx = (bid - nlines) * W + (threadIdx.x >> 6);
         for (int s = 0; s < 4; ++s) {
             if (idx < irr.count[s]) {
@@ -798,8 +831,8 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
     const int n = xdir ? p.nx : p.ny;
-    const float *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
-    float *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
+    const real *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
     cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, COMP, COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
                         COMP ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
                         COMP ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
@@ -807,7 +840,7 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
 
 // ------------------------------------------------------------------ boundaries
 
-struct Val { float w, hu, hv; };
+struct Val { real w, hu, hv; };
 
 __device__ __forceinline__ bool in_strip(const BoundaryParams &p, int side, int x, int y)
 {
@@ -899,19 +932,19 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, 
             if (sd == kWest || sd == kEast) v.hu = -v.hu; else v.hv = -v.hv;
             break;
         case kSponge: {
-            const float gamma = S.tab[strip_index(p, sd, ox, oy)];
+            const real gamma = S.tab[strip_index(p, sd, ox, oy)];
             v.w = gamma * (v.w - S.sl) + S.sl;
             v.hu = gamma * v.hu;
             v.hv = gamma * v.hv;
             break;
         }
         case kSine: {
-            const float *tb = S.tab + (size_t)strip_index(p, sd, ox, oy) * 5;
-            float e = 0.f, hu = 0.f, hv = 0.f;
+            const real *tb = S.tab + (size_t)strip_index(p, sd, ox, oy) * 5;
+            real e = 0.f, hu = 0.f, hv = 0.f;
             if (tb[4] != 0.f) {
-                const float t = p.clock ? p.clock[0] : p.t;
-                const float arg = S.omega * t - tb[0] - tb[1] + 0.f;
-                e = S.amp * sinf(arg);
+                const real t = p.clock ? p.clock[0] : p.t;
+                const real arg = S.omega * t - tb[0] - tb[1] + 0.f;
+                e = S.amp * SIN(arg);
                 hu = tb[2] * e;
                 hv = tb[3] * e;
             }
@@ -920,14 +953,14 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, 
         }
         case kUTS:
             if (p.clock) {
-                const float *c = p.clock + 1 + 3 * sd;
+                const real *c = p.clock + 1 + 3 * sd;
                 v.w = c[0] + S.sl; v.hu = c[1]; v.hv = c[2];
             } else {
                 v.w = S.uts_eta + S.sl; v.hu = S.uts_hu; v.hv = S.uts_hv;
             }
             break;
         default: {   // irregular
-            const float *q = S.irr + (size_t)strip_index(p, sd, ox, oy) * 3;
+            const real *q = S.irr + (size_t)strip_index(p, sd, ox, oy) * 3;
             v.w = q[0]; v.hu = q[1]; v.hv = q[2];
             break;
         }
@@ -949,6 +982,7 @@ __global__ __launch_bounds__(256) void boundary_apply(const BoundaryParams p)
     if (tid < p.ncells) boundary_cell(p, tid, p.mirror_pass);
 }
 
+#ifndef CEL_FP64   // batches run in the reference's precision only
 // ------------------------------------------------------------------ batched scenarios
 //
 // Many independent scenarios advance in one launch per kernel: the launch's blocks (or
@@ -1017,8 +1051,8 @@ __global__ __launch_bounds__(64 * W) void solve_batched(const SolveJob *__restri
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
     const int n = xdir ? p.nx : p.ny;
-    const float *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
-    float *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
+    const real *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
     const bool comp = J.comp != 0;
     cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
                         comp ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
@@ -1035,6 +1069,8 @@ __global__ __launch_bounds__(256) void boundary_batched(const BoundaryJob *__res
     boundary_cell(jobs[j].p, tid - starts[j], mirror_pass);
 }
 
+#endif
+
 // ------------------------------------------------------------------ diagnostics
 
 // GetStats, compute.hlsl:1319-1378: per-cell values in FP32 exactly as the shader forms
@@ -1046,15 +1082,15 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ __forceinline__ float wave_max(float v)
+__device__ __forceinline__ real wave_max(real v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = FMAX(v, __shfl_xor(v, o, 64));
     return v;
 }
 
 __global__ __launch_bounds__(256) void stats_partial(const StatsParams p)
 {
-    __shared__ float hvs[kStatsTileY][kStatsTileX + 1];
+    __shared__ real hvs[kStatsTileY][kStatsTileX + 1];
     __shared__ double red[4][kStatsValues];
     const int tid = threadIdx.x;
     const int tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
@@ -1067,29 +1103,29 @@ __global__ __launch_bounds__(256) void stats_partial(const StatsParams p)
     }
     __syncthreads();
     double sh = 0, sb = 0, shu = 0, shv = 0, sk = 0;
-    float mu = 0.f, mh = 0.f, mc = 0.f, mf = 0.f;
+    real mu = 0.f, mh = 0.f, mc = 0.f, mf = 0.f;
     const int xl = tid % kStatsTileX;
     for (int yl = tid / kStatsTileX; yl < kStatsTileY; yl += 256 / kStatsTileX) {
         const int x = x0 + xl, y = y0 + yl;
         if (x >= 2 + p.ncx || y >= 2 + p.ncy) continue;
         const size_t i = (size_t)y * p.P + x;
-        const float w = p.w[i], hu = p.hu[i], hv = hvs[yl][xl];
-        const float B = p.BA[i];
-        const float h = fmaxf(0.f, w - B);
-        const float c = sqrtf(p.g * h);
-        const float dbh = divide_by_h(h, p.epsilon);
-        const float u = dbh * hu, v = dbh * hv;
+        const real w = p.w[i], hu = p.hu[i], hv = hvs[yl][xl];
+        const real B = p.BA[i];
+        const real h = FMAX(real(0), w - B);
+        const real c = SQRT(p.g * h);
+        const real dbh = divide_by_h(h, p.epsilon);
+        const real u = dbh * hu, v = dbh * hv;
         sh += (double)h;
         sb += (double)(h * (B + 0.5f * h));
         shu += (double)hu;
         shv += (double)hv;
         sk += (double)(hu * u + hv * v);
-        const float u2v2 = u * u + v * v;
-        mu = fmaxf(mu, u2v2);
-        mh = fmaxf(mh, h);
-        mc = fmaxf(mc, (fabsf(u) + c) * p.one_over_dx);
-        mc = fmaxf(mc, (fabsf(v) + c) * p.one_over_dy);
-        mf = fmaxf(mf, u2v2 * dbh);
+        const real u2v2 = u * u + v * v;
+        mu = FMAX(mu, u2v2);
+        mh = FMAX(mh, h);
+        mc = FMAX(mc, (FABS(u) + c) * p.one_over_dx);
+        mc = FMAX(mc, (FABS(v) + c) * p.one_over_dy);
+        mf = FMAX(mf, u2v2 * dbh);
     }
     const double vals[kStatsValues] = {wave_sum(sh), wave_sum(sb), wave_sum(shu), wave_sum(shv), wave_sum(sk),
                                        (double)wave_max(mu), (double)wave_max(mh), (double)wave_max(mc),
@@ -1128,9 +1164,9 @@ __global__ __launch_bounds__(64) void stats_final(const StatsParams p)
 // ------------------------------------------------------------------ utilities
 
 // out[j * pout + i] = in[i * pin + j] for i < rows, j < cols (host-side layout changes)
-__global__ void transpose_plane(const float *in, int pin, float *out, int pout, int rows, int cols)
+__global__ void transpose_plane(const real *in, int pin, real *out, int pout, int rows, int cols)
 {
-    __shared__ float tile[32][33];
+    __shared__ real tile[32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     for (int j = threadIdx.y; j < 32; j += 8) {
         const int r = by + j, c = bx + threadIdx.x;
@@ -1150,20 +1186,20 @@ __global__ void static_fields(const StaticParams p)
     const int x = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int y = 2 + blockIdx.y;
     if (x >= p.nx + 2 || y >= p.ny + 2) return;
-    const float *BA = p.BA;
+    const real *BA = p.BA;
     auto ba = [&](int xx, int yy) { return BA[(size_t)yy * p.P + xx]; };
-    const float cs = p.coef_sl, sl = p.sea_level;
-    float Kx, Qx, Ky, Qy;
+    const real cs = p.coef_sl, sl = p.sea_level;
+    real Kx, Qx, Ky, Qy;
     line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ba(x - 2, y)), fmaxr(0.f, cs - ba(x - 1, y)), fmaxr(0.f, cs - ba(x, y)),
             fmaxr(0.f, cs - ba(x + 1, y)), fmaxr(0.f, cs - ba(x + 2, y)), Kx, Qx);
     line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - ba(x, y - 2)), fmaxr(0.f, cs - ba(x, y - 1)), fmaxr(0.f, cs - ba(x, y)),
             fmaxr(0.f, cs - ba(x, y + 1)), fmaxr(0.f, cs - ba(x, y + 2)), Ky, Qy);
-    const float d_l = fmaxr(0.f, sl - ba(x - 1, y)), d_r = fmaxr(0.f, sl - ba(x + 1, y));
-    const float d_ll = fmaxr(0.f, sl - ba(x - 2, y)), d_rr = fmaxr(0.f, sl - ba(x + 2, y));
-    const float d_d = fmaxr(0.f, sl - ba(x, y - 1)), d_u = fmaxr(0.f, sl - ba(x, y + 1));
-    const float d_dd = fmaxr(0.f, sl - ba(x, y - 2)), d_uu = fmaxr(0.f, sl - ba(x, y + 2));
-    const float ddx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * p.one_over_dx / 12.f;
-    const float ddy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * p.one_over_dy / 12.f;
+    const real d_l = fmaxr(0.f, sl - ba(x - 1, y)), d_r = fmaxr(0.f, sl - ba(x + 1, y));
+    const real d_ll = fmaxr(0.f, sl - ba(x - 2, y)), d_rr = fmaxr(0.f, sl - ba(x + 2, y));
+    const real d_d = fmaxr(0.f, sl - ba(x, y - 1)), d_u = fmaxr(0.f, sl - ba(x, y + 1));
+    const real d_dd = fmaxr(0.f, sl - ba(x, y - 2)), d_uu = fmaxr(0.f, sl - ba(x, y + 2));
+    const real ddx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * p.one_over_dx / 12.f;
+    const real ddy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * p.one_over_dy / 12.f;
     const size_t r = (size_t)y * p.P + (x - 2), c = (size_t)x * p.PT + (y - 2);
     p.Kx[r] = Kx; p.Qx[r] = Qx; p.Ky[r] = Ky; p.Qy[r] = Qy; p.DDx[r] = ddx; p.DDy[r] = ddy;
     p.KyT[c] = Ky; p.QyT[c] = Qy;
@@ -1172,14 +1208,14 @@ __global__ void static_fields(const StaticParams p)
 // level-0 coefficients of every cell for the x- and y-lines (diagnostic export): ghost rows
 // of the x-lines repeat the nearest interior row, ghost columns of the y-lines the nearest
 // interior column (engine.cpp:2641, 2797); line margins are (0, 1, 0).
-__global__ void export_coefficients(const StaticParams p, float *ax, float *ay)
+__global__ void export_coefficients(const StaticParams p, real *ax, real *ay)
 {
     const int X = p.nx + 4, Y = p.ny + 4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= X * Y) return;
     const int x = i % X, y = i / X;
     const size_t N = (size_t)X * Y;
-    float A = 0.f, B = 1.f, C = 0.f;
+    real A = 0.f, B = 1.f, C = 0.f;
     if (x >= 2 && x <= X - 3) {
         const int yy = y < 2 ? 2 : (y > p.ny + 1 ? p.ny + 1 : y);
         const size_t r = (size_t)yy * p.P + (x - 2);
@@ -1196,11 +1232,11 @@ __global__ void export_coefficients(const StaticParams p, float *ax, float *ay)
 }
 
 
-}  // namespace cel
+}  // namespace CEL_NS
 
 // ------------------------------------------------------------------ launchers (host)
 
-namespace cel {
+namespace CEL_NS {
 
 hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st)
 {
@@ -1291,6 +1327,7 @@ hipError_t launch_solve(bool comp, int shape_x, int shape_y, const LineParams &p
     return hipGetLastError();
 }
 
+#ifndef CEL_FP64
 hipError_t launch_explicit_batched(int rows, const ExplicitJob *jobs, const int *starts, int njobs, int total_tiles,
                                    hipStream_t st)
 {
@@ -1325,6 +1362,8 @@ hipError_t launch_boundary_batched(const BoundaryJob *jobs, const int *starts, i
     return hipGetLastError();
 }
 
+#endif
+
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
 {
     if (p.ncells == 0) return hipSuccess;
@@ -1346,11 +1385,11 @@ hipError_t launch_stats(const StatsParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_export_coefficients(const StaticParams &p, float *ax, float *ay, hipStream_t st)
+hipError_t launch_export_coefficients(const StaticParams &p, real *ax, real *ay, hipStream_t st)
 {
     const int n = (p.nx + 4) * (p.ny + 4);
     hipLaunchKernelGGL(export_coefficients, dim3((n + 255) / 256), dim3(256), 0, st, p, ax, ay);
     return hipGetLastError();
 }
 
-}  // namespace cel
+}  // namespace CEL_NS

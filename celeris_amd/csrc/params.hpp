@@ -1,14 +1,35 @@
 // params.hpp -- launch parameter blocks shared by the host engine and the HIP kernels.
 //
-// Device memory layout (structure of arrays, FP32):
+// Included once per working precision: namespace cel with real = float (the reference's
+// precision) and, when CEL_FP64 is defined, namespace cel64 with real = double.
+//
+// Device memory layout (structure of arrays of `real`):
 //   row-major planes    : element (x, y) at [y * P + x],  P  = padded row pitch  (>= nx+4+16)
 //   transposed planes   : element (x, y) at [x * PT + y], PT = padded column pitch (>= ny+4+16)
 // x and y are ghost-inclusive indices, interior cells are [2, nx+2) x [2, ny+2).
 // hv and everything feeding the y-line solve live transposed so that both line
 // solves read contiguous memory.
-#pragma once
+#undef CEL_NS
+#ifdef CEL_FP64
+#define CEL_NS cel64
+#else
+#define CEL_NS cel
+#endif
 
-namespace cel {
+#if (defined(CEL_FP64) && !defined(CEL_PARAMS_HPP_FP64)) || (!defined(CEL_FP64) && !defined(CEL_PARAMS_HPP_FP32))
+#ifdef CEL_FP64
+#define CEL_PARAMS_HPP_FP64
+#else
+#define CEL_PARAMS_HPP_FP32
+#endif
+
+namespace CEL_NS {
+
+#ifdef CEL_FP64
+using real = double;
+#else
+using real = float;
+#endif
 
 // Rows one wave marches through in the explicit kernel: 4, 8 or 16 (chosen per engine by
 // the autotuner; 8 by default).
@@ -24,53 +45,53 @@ struct ExplicitParams {
     int nx, ny, P, PT;
     int nstrips, nbands;
     // state at step n
-    const float *w, *hu, *hvT;
+    const real *w, *hu, *hvT;
     // compensated state: low word of w at n (read) and n+1 (written)
-    const float *w_lo;
-    float *w_lo_new;
+    const real *w_lo;
+    real *w_lo_new;
     // bottom: BY (north face), BX (east face), BA (centre)
-    const float *BY, *BX, *BA;
+    const real *BY, *BX, *BA;
     // Adams-Bashforth history: dU/dt at n-1 and n-2 (w, hu, hv planes), (F*, G*) at n-1, n-2
-    const float *Go[3], *Goo[3], *FGo[2], *FGoo[2];
+    const real *Go[3], *Goo[3], *FGo[2], *FGoo[2];
     // corrector inputs: predicted dU/dt and (F*, G*), and the state the predictor started from
-    const float *Gp[3], *FGp[2], *Cw, *Chu, *ChvT;
+    const real *Gp[3], *FGp[2], *Cw, *Chu, *ChvT;
     // outputs
-    float *w_new;    // w^{n+1} (interior)
-    float *Dx;       // right-hand side of the x-line systems, [y * P + (x - 2)]
-    float *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
-    float *Gs[3], *FGs[2];
+    real *w_new;    // w^{n+1} (interior)
+    real *Dx;       // right-hand side of the x-line systems, [y * P + (x - 2)]
+    real *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
+    real *Gs[3], *FGs[2];
     // SimConstBuffer (compute.hlsl:65-94)
-    float two_theta, g, half_g, Bcoef_g, g_over_dx, g_over_dy, one_over_dx, one_over_dy;
-    float dt, epsilon, friction, sea_level;
+    real two_theta, g, half_g, Bcoef_g, g_over_dx, g_over_dy, one_over_dx, one_over_dy;
+    real dt, epsilon, friction, sea_level;
     int is_manning;
     // variable-step AB3 coefficients (AdamsBashforthCoef) and derivative weights
-    float ab[3];
-    float der[9];
+    real ab[3];
+    real der[9];
     // static per-cell fields, [y * P + (x - 2)]: K1 and K2/72 of the x- and y-line
     // coefficients, and the fourth-order depth gradients dd/dx, dd/dy
-    const float *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;
+    const real *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;
 };
 
 // Static per-cell fields derived from the bottom once per mesh (bit-identical to computing
 // them inside the step: same operations on the same inputs).
 struct StaticParams {
     int nx, ny, P, PT;
-    const float *BA;
-    float *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;   // row-major, [y * P + (x - 2)]
-    float *KyT, *QyT;                       // transposed, [x * PT + (y - 2)]
-    float kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
-    float sea_level, one_over_dx, one_over_dy;
+    const real *BA;
+    real *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;   // row-major, [y * P + (x - 2)]
+    real *KyT, *QyT;                       // transposed, [x * PT + (y - 2)]
+    real kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
+    real sea_level, one_over_dx, one_over_dy;
 };
 
 struct LineParams {
     int nx, ny, P, PT;
     int nlines_x, nlines_y;         // blocks [0, nlines_x) solve x-lines, then y-lines
-    const float *Dx, *Kx, *Qx;      // x-lines: rhs and coefficient fields, [y * P + (x - 2)]
-    const float *DyT, *KyT, *QyT;   // y-lines: the same, transposed, [x * PT + (y - 2)]
-    float *hu_out, *hvT_out;
+    const real *Dx, *Kx, *Qx;      // x-lines: rhs and coefficient fields, [y * P + (x - 2)]
+    const real *DyT, *KyT, *QyT;   // y-lines: the same, transposed, [x * PT + (y - 2)]
+    real *hu_out, *hvT_out;
     // compensated state: previous hi/lo words and the new low words (same layouts as out)
-    const float *hu_prev, *hu_prev_lo, *hvT_prev, *hvT_prev_lo;
-    float *hu_lo_out, *hvT_lo_out;
+    const real *hu_prev, *hu_prev_lo, *hvT_prev, *hvT_prev_lo;
+    real *hu_lo_out, *hvT_lo_out;
     int Lx, Ly;                     // cyclic-reduction level counts
     // irregular-wave strips, handled by blocks after the line solves
     int n_irr_lines;                // total (side, strip-index, line) triples
@@ -87,12 +108,12 @@ struct IrregularSide {
     int nleaf_pow2;      // smallest power of two >= nleaf_active (<= 2^(L-1))
     int levels_above;    // log2(2^(L-1) / nleaf_pow2)
     int zero_line;       // 1 if the reference copies an all-zero texture (L == 1)
-    float sl;
-    const float *amp, *omega, *phase;    // per component
-    const float *tab;                    // [(strip * nlines + line) * nw + wave] * 4: kx, ky, c*cos, c*sin
+    real sl;
+    const real *amp, *omega, *phase;    // per component
+    const real *tab;                    // [(strip * nlines + line) * nw + wave] * 4: kx, ky, c*cos, c*sin
     const unsigned char *wet;            // [strip * nlines + line]: d > MIN_DEPTH
     int nw;                              // components stored per line
-    float *out;                          // [(strip * nlines + line) * 3]
+    real *out;                          // [(strip * nlines + line) * 3]
 };
 
 // Step clock: per-step values that change every step while the launch pattern does not.
@@ -103,17 +124,17 @@ constexpr int kClockStride = 16;
 struct IrregularAll {
     IrregularSide side[4];
     int count[4];      // width * nlines per side (0 if inactive)
-    float t;
-    const float *clock;   // if set, t = clock[0]
+    real t;
+    const real *clock;   // if set, t = clock[0]
 };
 
 struct SideParams {
     int type, width;
-    float sl;
-    const float *tab;          // sponge: gamma; sine: kx, ky, c*cos, c*sin, wet (5 floats) per strip cell
-    float amp, omega;          // sine
-    float uts_eta, uts_hu, uts_hv;
-    const float *irr;          // irregular: strip values from IrregularSide::out
+    real sl;
+    const real *tab;          // sponge: gamma; sine: kx, ky, c*cos, c*sin, wet (5 floats) per strip cell
+    real amp, omega;          // sine
+    real uts_eta, uts_hu, uts_hv;
+    const real *irr;          // irregular: strip values from IrregularSide::out
 };
 
 struct BoundaryParams {
@@ -122,10 +143,10 @@ struct BoundaryParams {
     int band_l, band_r, band_b, band_t;   // widths of the boundary cell set (>= 2 each)
     int ncells;
     int mirror_pass;                      // 1: cells whose chain passes a solid side; 0: all others
-    float t;
-    const float *clock;                   // if set, t and the time-series values come from it
-    float *w, *hu, *hvT;
-    float *w_lo, *hu_lo, *hvT_lo;         // compensated state: low words (zeroed where set)
+    real t;
+    const real *clock;                   // if set, t and the time-series values come from it
+    real *w, *hu, *hvT;
+    real *w_lo, *hu_lo, *hvT_lo;         // compensated state: low words (zeroed where set)
     SideParams side[4];
 };
 
@@ -149,10 +170,12 @@ constexpr int kStatsTileX = 64, kStatsTileY = 16, kStatsValues = 9;
 struct StatsParams {
     int ncx, ncy, P, PT;
     int tiles_x, nblocks;
-    const float *w, *hu, *hvT, *BA;
-    float g, one_over_dx, one_over_dy, epsilon;
+    const real *w, *hu, *hvT, *BA;
+    real g, one_over_dx, one_over_dy, epsilon;
     double *partial;   // [block * kStatsValues + v]
     double *out;       // kStatsValues
 };
 
-}  // namespace cel
+}  // namespace CEL_NS
+
+#endif

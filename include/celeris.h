@@ -52,14 +52,16 @@ typedef enum {
     CEL_PREDICTOR_ADAPTIVE = 3
 } cel_time_scheme;
 
-/* State storage and update.  CEL_STATE_FP32 is the reference's arithmetic.
+/* State storage and arithmetic.  CEL_STATE_FP32 is the reference's arithmetic.
  * CEL_STATE_COMPENSATED keeps (w, hu, hv) as two-float pairs (hi + lo): the explicit
  * update adds its increment with an exact two-sum, and the line systems are solved for
  * the increment of hu and hv, which is then added the same way, so increments far below
  * the state's FP32 resolution accumulate instead of rounding away.  Every other
- * operation reads the hi words in FP32.  Not combined with corrector steps.  No
- * counterpart in the reference. */
-typedef enum { CEL_STATE_FP32 = 0, CEL_STATE_COMPENSATED = 1 } cel_state_precision;
+ * operation reads the hi words in FP32.  Not combined with corrector steps.
+ * CEL_STATE_FP64 runs every kernel and every host-side table in FP64 (the same
+ * operations in double precision); arrays still cross the ABI as FP32 except through
+ * cel_engine_get_state_f64.  Not available in batches.  No counterpart in the reference. */
+typedef enum { CEL_STATE_FP32 = 0, CEL_STATE_COMPENSATED = 1, CEL_STATE_FP64 = 2 } cel_state_precision;
 
 /* Maximum irregular-wave components per side: MAX_NUM_OF_IRREGULAR_WAVES,
  * engine.hpp:216 / compute.hlsl:111-114. */

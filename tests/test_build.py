@@ -11,10 +11,10 @@ import pytest
 from celeris_amd import build as B
 
 
-@pytest.fixture(scope="module")
-def resource_report(tmp_path_factory):
+def _report(tmp_path_factory, defines):
     out = tmp_path_factory.mktemp("res") / "k.o"
-    cmd = [os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")] + [f for f in B.FLAGS if f not in ("-shared", "-fPIC")]
+    cmd = [os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")] + [f for f in B.COMPILE_FLAGS if f != "-fPIC"]
+    cmd += [f"-D{d}" for d in defines]
     cmd += ["--cuda-device-only", "-c", os.path.join(B.CSRC, "kernels.hip"), "-o", str(out),
             "-Rpass-analysis=kernel-resource-usage"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -27,10 +27,26 @@ def resource_report(tmp_path_factory):
     return kernels
 
 
+@pytest.fixture(scope="module")
+def resource_report(tmp_path_factory):
+    return _report(tmp_path_factory, [])
+
+
+@pytest.fixture(scope="module")
+def resource_report_fp64(tmp_path_factory):
+    return _report(tmp_path_factory, ["CEL_FP64"])
+
+
 def test_no_kernel_uses_scratch(resource_report):
     assert len(resource_report) >= 10
     bad = {k: v for k, v in resource_report.items() if v.get("ScratchSize", 0) or v.get("VGPRs", 0)}
     assert not bad, f"kernels with scratch or spills: {bad}"
+
+
+def test_fp64_kernels_use_no_scratch(resource_report_fp64):
+    assert len(resource_report_fp64) >= 10
+    bad = {k: v for k, v in resource_report_fp64.items() if v.get("ScratchSize", 0) or v.get("VGPRs", 0)}
+    assert not bad, f"FP64 kernels with scratch or spills: {bad}"
 
 
 def test_explicit_step_occupancy(resource_report):
