@@ -124,3 +124,45 @@ def test_precision_study_solitary_beach():
     print("eta error / H (Linf, L2):", {k: ("%.2e" % v[0], "%.2e" % v[1]) for k, v in errs.items()})
     assert errs["fp64"][0] < 1e-10
     assert errs["fp32"][0] < 1e-3 and errs["compensated"][0] < 1e-3
+
+
+def _runup(state, bottom, row):
+    """Highest bed elevation reached by water (h > 1e-3 m) along one row."""
+    h = state[0][row] - bottom[2][row]
+    wet = h > 1e-3
+    return float(bottom[2][row][wet].max()) if wet.any() else float("nan")
+
+
+@pytest.mark.slow
+def test_precision_study_reduced_c3():
+    """The precision case on a 2048 x 8 channel (dx 0.25 m, 1:50 beach to +0.5 m, solitary wave
+    H/h = 0.3 from the west, walls all round), 24000 steps: the wave crosses the channel and
+    runs up the beach.  Reports eta error (L-inf, L2, over H) against the FP64 oracle, the
+    runup height and the volume drift of each variant."""
+    sc = S.solitary_channel(nx=2048, ny=8, dx=0.25, steps=24000)
+    H, row, chunk = 0.3, 6, 2000
+    o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f64")
+    v0 = o.stats()[0]
+    ref_runup = 0.0
+    for _ in range(sc.steps // chunk):
+        o.step(chunk)
+        ref_runup = max(ref_runup, _runup(o.state(), sc.bottom, row))
+    ref = o.state()
+    out = {"oracle": {"runup_m": ref_runup, "volume_drift": (o.stats()[0] - v0) / v0}}
+    for name, prec in (("fp32", STATE_FP32), ("compensated", STATE_COMPENSATED), ("fp64", STATE_FP64)):
+        sc.s.state_precision = prec
+        e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        g0 = e.stats()["sum_h"]
+        runup = 0.0
+        for _ in range(sc.steps // chunk):
+            e.timestep(chunk)
+            runup = max(runup, _runup(e.get_state_f64(), sc.bottom, row))
+        a = e.get_state_f64()
+        d = (a[0] - ref[0])[2:-2, 2:-2]
+        out[name] = {"eta_linf_over_H": float(np.abs(d).max() / H), "eta_l2_over_H": float(np.sqrt((d ** 2).mean()) / H),
+                     "runup_m": runup, "volume_drift": (e.stats()["sum_h"] - g0) / g0}
+        e.close()
+    print("precision study (2048x8, 24000 steps):", out)
+    assert out["fp64"]["eta_linf_over_H"] < 1e-9
+    assert out["fp32"]["eta_linf_over_H"] < 1e-2 and out["compensated"]["eta_linf_over_H"] < 1e-2
+    assert abs(out["fp64"]["runup_m"] - ref_runup) < 1e-9
