@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 BYTES_PER_CELL_STEP = 96          # algorithmic HBM bytes per interior cell-step (plain FP32)
+# per state precision: the compensated state adds 12 B read + 12 B written (the low words),
+# FP64 doubles every plane
+PRECISIONS = {"fp32": (0, 96, "f32"), "compensated": (1, 120, "f32 + f32 (compensated)"), "fp64": (2, 192, "f64")}
 HBM_PEAK_GBS = 8000.0             # MI355X HBM3E peak
 # measured HBM bytes per explicit_step launch at 2048^2 (rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # scaled by the copy calibration) and the same kernel's VALU issue utilisation
@@ -146,6 +149,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=10,
                     help="repeats per candidate when autotuning band height and line-solve shapes (0: defaults)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
+                    help="state precision of the headline workload (the headline figure is fp32)")
     ap.add_argument("--workload", choices=["headline", "ensemble"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
@@ -185,6 +190,8 @@ def main():
     from celeris_amd import scenarios as S
 
     sc = S.headline(nx=args.nx, ny=args.ny)
+    prec_code, bytes_per_cell, dtype = PRECISIONS[args.precision]
+    sc.s.state_precision = prec_code
     eng = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     cells = sc.s.nx * sc.s.ny
     eng.set_graph_steps(args.graph_steps)
@@ -212,8 +219,8 @@ def main():
     eng.close()
 
     avg_ms = kern["explicit_step"]
-    achieved = BYTES_PER_CELL_STEP * cells / (avg_ms * 1e-3) / 1e9
-    traffic, valu_busy = measured_traffic(sc.s.nx, sc.s.ny)
+    achieved = bytes_per_cell * cells / (avg_ms * 1e-3) / 1e9
+    traffic, valu_busy = measured_traffic(sc.s.nx, sc.s.ny) if args.precision == "fp32" else (None, None)
     out = {
         "metric": "Gcell-steps/s (2048^2 Boussinesq)",
         "value": value,
@@ -225,23 +232,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": f"Boussinesq step, {sc.s.nx}x{sc.s.ny} interior cells, dx 1 m, 1:30 slope + GRF, "
-                               "west irregular waves (100 JONSWAP components), 20-cell east sponge, N/S walls, FP32",
+                               "west irregular waves (100 JONSWAP components), 20-cell east sponge, N/S walls, "
+                               + args.precision.upper() + " state",
                    "nx": sc.s.nx, "ny": sc.s.ny, "dt": sc.s.timestep, "graph_steps": args.graph_steps,
                    "tuning": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in tuning.items()},
-                   "hbm_frac_step": value * 1e9 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
+                   "hbm_frac_step": value * 1e9 * bytes_per_cell / (HBM_PEAK_GBS * 1e9),
                    "event_ms_per_step": ms_events / args.steps,
                    "kernel_avg_ms": kern, "state_finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "explicit_step",
-                     "bytes_per_launch": BYTES_PER_CELL_STEP * cells,
+                     "bytes_per_launch": bytes_per_cell * cells,
                      "measured_hbm_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9,
                      "valu_busy": valu_busy},
         "cpu_baseline": None,
     }
-    print(f"bench: 2048^2 step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
+    print(f"bench: {sc.s.nx}x{sc.s.ny} {args.precision} step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
     if args.serving_requests > 0:
         rates = [float(v) for v in args.serving_rates.split(",")]
         out["serving"] = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)

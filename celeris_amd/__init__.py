@@ -69,6 +69,14 @@ SIGNATURES = {
     "cel_last_error": ([], ctypes.c_char_p),
     "cel_bottom_from_corners": ([ctypes.c_int, ctypes.c_int, _F, _F], ctypes.c_int),
     "cel_initial_state": ([ctypes.POINTER(CelSettings), _F, ctypes.POINTER(CelSoliton), ctypes.c_int, _F], ctypes.c_int),
+    "cel_hot_start_state": ([ctypes.POINTER(CelSettings), _F, _F, ctypes.POINTER(CelSoliton), ctypes.c_int, _F],
+                            ctypes.c_int),
+    "cel_engine_sample": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _F], ctypes.c_int),
+    "cel_engine_timestep_count": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_engine_set_visual": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float], ctypes.c_int),
+    "cel_engine_get_aux": ([ctypes.c_void_p, _F], ctypes.c_int),
+    "cel_engine_get_normals": ([ctypes.c_void_p, _F], ctypes.c_int),
+    "cel_engine_add_solitary_wave": ([ctypes.c_void_p] + [ctypes.c_float] * 4, ctypes.c_int),
     "cel_engine_create": ([ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
                            ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "cel_engine_destroy": ([ctypes.c_void_p], ctypes.c_int),
@@ -191,6 +199,17 @@ def initial_state(s, bottom, solitons=()):
     return out
 
 
+def hot_start_state(s, bottom, hot, solitons=()):
+    """Initial condition from a hot-start state plus solitary waves (engine.cpp:2943-3016)."""
+    bottom = np.ascontiguousarray(bottom, dtype=np.float32)
+    hot = np.ascontiguousarray(hot, dtype=np.float32)
+    out = np.zeros((3, s.ny + 4, s.nx + 4), np.float32)
+    arr = (CelSoliton * max(1, len(solitons)))(*[CelSoliton(*sol) for sol in solitons])
+    _check(load_library().cel_hot_start_state(ctypes.byref(s), _fptr(bottom), _fptr(hot), arr, len(solitons),
+                                              _fptr(out)))
+    return out
+
+
 class ShallowWaterEngine:
     """The step, settings and boundary interface of ShallowWaterEngine (engine.hpp:58-97) on one GPU."""
 
@@ -252,6 +271,15 @@ class ShallowWaterEngine:
         state = np.ascontiguousarray(state, dtype=np.float32)
         _check(load_library().cel_engine_set_state(self._h, _fptr(state)))
 
+    def sample(self, cells):
+        """(w, hu, hv) at ghost-inclusive cells [(i, j), ...], read on the device: shape (n, 3)."""
+        xy = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros((len(xy), 3), np.float32)
+        if len(xy):
+            _check(load_library().cel_engine_sample(self._h, xy.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                     len(xy), _fptr(out)))
+        return out
+
     @property
     def time(self):
         t = ctypes.c_double()
@@ -263,6 +291,31 @@ class ShallowWaterEngine:
         v = ctypes.c_int()
         _check(load_library().cel_engine_scheme(self._h, ctypes.byref(v)))
         return v.value
+
+    @property
+    def timestep_count(self):
+        """The reference's timestep_count: 1 at creation, +1 per completed predictor cycle."""
+        v = ctypes.c_int()
+        _check(load_library().cel_engine_timestep_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_visual(self, enable=True, dissipation_threshold=0.25, whitewater_decay=0.1):
+        """Turns the per-step visual outputs (inundation record, slope angle, whitewater) on/off."""
+        _check(load_library().cel_engine_set_visual(self._h, int(enable), dissipation_threshold, whitewater_decay))
+
+    def aux(self):
+        """(max inundated depth, slope angle, whitewater) planes, ghost-inclusive."""
+        out = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)
+        _check(load_library().cel_engine_get_aux(self._h, _fptr(out)))
+        return out
+
+    def normals(self):
+        out = np.zeros((3, self.ny + 4, self.nx + 4), np.float32)
+        _check(load_library().cel_engine_get_normals(self._h, _fptr(out)))
+        return out
+
+    def add_solitary_wave(self, H, theta_deg, x0, y0):
+        _check(load_library().cel_engine_add_solitary_wave(self._h, H, theta_deg, x0, y0))
 
     def timed_steps(self, n):
         ms = ctypes.c_float()

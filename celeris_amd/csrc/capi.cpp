@@ -12,7 +12,8 @@
 
 namespace cel {
 void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
-void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3);
+void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3,
+                   const float *base3);
 }
 
 struct cel_engine {
@@ -80,8 +81,52 @@ int cel_initial_state(const cel_settings *s, const float *bottom3, const cel_sol
         if (!s || !bottom3 || !state3 || n_solitons < 0 || (n_solitons > 0 && !solitons))
             throw cel::Error(CEL_EINVAL, "bad arguments");
         if (s->nx < 4 || s->ny < 4) throw cel::Error(CEL_EINVAL, "nx and ny must be at least 4");
-        cel::initial_state(*s, bottom3, solitons, n_solitons, state3);
+        cel::initial_state(*s, bottom3, solitons, n_solitons, state3, nullptr);
     });
+}
+
+int cel_hot_start_state(const cel_settings *s, const float *bottom3, const float *hot3, const cel_soliton *solitons,
+                        int n_solitons, float *state3)
+{
+    return guarded([&] {
+        if (!s || !bottom3 || !hot3 || !state3 || n_solitons < 0 || (n_solitons > 0 && !solitons))
+            throw cel::Error(CEL_EINVAL, "bad arguments");
+        if (s->nx < 4 || s->ny < 4) throw cel::Error(CEL_EINVAL, "nx and ny must be at least 4");
+        cel::initial_state(*s, bottom3, solitons, n_solitons, state3, hot3);
+    });
+}
+
+int cel_engine_sample(cel_engine *e, const int *xy, int n, float *out3)
+{
+    return guarded([&] { eng(e).sample(xy, n, out3); });
+}
+
+int cel_engine_timestep_count(cel_engine *e, int *count)
+{
+    return guarded([&] {
+        if (!count) throw cel::Error(CEL_EINVAL, "null output");
+        *count = eng(e).timestepCount();
+    });
+}
+
+int cel_engine_set_visual(cel_engine *e, int enable, float dissipation_threshold, float whitewater_decay)
+{
+    return guarded([&] { eng(e).setVisual(enable != 0, dissipation_threshold, whitewater_decay); });
+}
+
+int cel_engine_get_aux(cel_engine *e, float *aux3)
+{
+    return guarded([&] { eng(e).getAux(aux3); });
+}
+
+int cel_engine_get_normals(cel_engine *e, float *normal3)
+{
+    return guarded([&] { eng(e).getNormals(normal3); });
+}
+
+int cel_engine_add_solitary_wave(cel_engine *e, float H, float theta_deg, float x0, float y0)
+{
+    return guarded([&] { eng(e).addSolitaryWave(H, theta_deg, x0, y0); });
 }
 
 int cel_engine_create(const cel_settings *s, const cel_boundary bnd[4], const float *bottom3, const float *state3,

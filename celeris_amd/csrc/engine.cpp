@@ -100,6 +100,7 @@ ShallowWaterEngine::~ShallowWaterEngine()
     if (arena_) (void)hipFree(arena_);
     if (d_stats_) (void)hipFree(d_stats_);
     if (h_stats_) (void)hipHostFree(h_stats_);
+    if (aux_mem_) (void)hipFree(aux_mem_);
     clearGraphs();
     if (d_clock_) (void)hipFree(d_clock_);
     for (auto &h : h_clock_) if (h) (void)hipHostFree(h);
@@ -235,6 +236,7 @@ void ShallowWaterEngine::setState(const float *state3)
             check(hipMemsetAsync(hvT_lo_[k], 0, tplane, stream_), "zero low words");
         }
     }
+    if (aux_on_) zeroAux();
     check(hipStreamSynchronize(stream_), "upload sync");
     resetScheme();
 }
@@ -625,6 +627,8 @@ void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
     int keep_uts[4];
     for (int s = 0; s < 4; ++s) keep_uts[s] = side_[s].uts_iter;
     fillExplicit(L.ep);
+    L.aux = aux_on_;
+    if (aux_on_) fillAux(L.ap);
     // predictor copies for a following corrector (engine.cpp:1050-1058); without corrector
     // steps nothing ever reads them
     if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
@@ -725,6 +729,8 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const real *dclock, bool
         for (auto &e : pe) e = nextEvent();
         check(hipEventRecord(pe[0], stream_), "event");
     }
+    // the visual outputs read only the step's starting state, which no kernel of the step writes
+    if (L.aux) check(launch_aux(L.ap, stream_), "aux_update");
     check(launch_explicit(L.scheme, comp_, L.rows, L.ep, stream_), "explicit_step");
     if (profile) check(hipEventRecord(pe[1], stream_), "event");
     for (int i = 0; i < L.ncopy; ++i)
@@ -1081,6 +1087,126 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
         tuned_us_[1 + dir] = shape_us[best_shape];
     }
     if (out) getTuning(out);
+}
+
+void ShallowWaterEngine::sample(const int *xy, int n, float *out3)
+{
+    if (n < 0 || (n > 0 && (!xy || !out3))) throw Error(CEL_EINVAL, "bad sample arguments");
+    if (n == 0) return;
+    for (int k = 0; k < n; ++k)
+        if (xy[2 * k] < 0 || xy[2 * k] >= X_ || xy[2 * k + 1] < 0 || xy[2 * k + 1] >= Y_)
+            throw Error(CEL_EINVAL, "sample cell outside the grid");
+    void *mem = nullptr;
+    const size_t out_off = ((size_t)n * 2 * sizeof(int) + 15) / 16 * 16;   // 16-byte aligned output
+    const size_t bytes = out_off + (size_t)n * 3 * sizeof(real);
+    check(hipMalloc(&mem, bytes), "hipMalloc(sample)");
+    int *dxy = static_cast<int *>(mem);
+    real *dout = reinterpret_cast<real *>(static_cast<char *>(mem) + out_off);
+    std::vector<real> h((size_t)3 * n);
+    hipError_t e = hipMemcpyAsync(dxy, xy, (size_t)n * 2 * sizeof(int), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = launch_gather(w_[cur_], hu_[cur_], hvT_[cur_], P_, PT_, dxy, n, dout, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, (size_t)n * 3 * sizeof(real), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(mem);
+    check(e, "sample");
+    for (size_t i = 0; i < h.size(); ++i) out3[i] = (float)h[i];
+}
+
+// ---- visual outputs (compute.hlsl:437-457, 535-540) and addSolitaryWave
+
+void ShallowWaterEngine::fillAux(AuxParams &p) const
+{
+    const size_t plane = (size_t)P_ * Y_;
+    p.nx = nx_; p.ny = ny_; p.P = P_;
+    p.w = w_[cur_]; p.BX = BX_; p.BY = BY_;
+    p.depth = aux_mem_; p.slope = aux_mem_ + plane; p.white = aux_mem_ + 2 * plane;
+    p.two_theta = real(2) * (real)s_.theta;
+    p.one_over_dx = real(1) / dx_;
+    p.one_over_dy = real(1) / dy_;
+    p.threshold = aux_threshold_;
+    // pow(abs(whiteWaterDecayRate), dt) is uniform over the grid: evaluated once per step
+    p.decay = std::pow(std::fabs(aux_rate_), current_timestep_);
+}
+
+void ShallowWaterEngine::zeroAux()
+{
+    check(hipMemsetAsync(aux_mem_, 0, (size_t)P_ * Y_ * 3 * sizeof(real), stream_), "zero aux");
+}
+
+void ShallowWaterEngine::setVisual(bool on, float threshold, float decay)
+{
+    sync();
+    clearGraphs();
+    if (on) {
+        if (!aux_mem_) {
+            void *mem = nullptr;
+            check(hipMalloc(&mem, (size_t)P_ * Y_ * 3 * sizeof(real)), "hipMalloc(aux)");
+            aux_mem_ = static_cast<real *>(mem);
+            table_bytes_ += (size_t)P_ * Y_ * 3 * sizeof(real);
+        }
+        zeroAux();
+        aux_threshold_ = (real)threshold;
+        aux_rate_ = (real)(1.0f / (1 + 2 * decay));     // engine.cpp:4129
+        check(hipStreamSynchronize(stream_), "sync");
+    } else if (aux_mem_) {
+        (void)hipFree(aux_mem_);
+        aux_mem_ = nullptr;
+        table_bytes_ -= (size_t)P_ * Y_ * 3 * sizeof(real);
+    }
+    aux_on_ = on;
+}
+
+void ShallowWaterEngine::getAux(float *aux3)
+{
+    if (!aux_on_) throw Error(CEL_EINVAL, "visual outputs are off");
+    if (!aux3) throw Error(CEL_EINVAL, "null output");
+    const size_t N = (size_t)X_ * Y_, e = sizeof(real);
+    std::vector<real> h(3 * N);
+    for (int c = 0; c < 3; ++c)
+        check(hipMemcpy2DAsync(h.data() + c * N, (size_t)X_ * e, aux_mem_ + (size_t)c * P_ * Y_, (size_t)P_ * e,
+                               (size_t)X_ * e, Y_, hipMemcpyDeviceToHost, stream_), "download aux");
+    check(hipStreamSynchronize(stream_), "sync");
+    for (size_t i = 0; i < 3 * N; ++i) aux3[i] = (float)h[i];
+}
+
+void ShallowWaterEngine::getNormals(float *normal3)
+{
+    if (!normal3) throw Error(CEL_EINVAL, "null output");
+    const size_t N = (size_t)X_ * Y_;
+    void *mem = nullptr;
+    check(hipMalloc(&mem, 3 * N * sizeof(real)), "hipMalloc(normals)");
+    NormalParams p;
+    p.X = X_; p.Y = Y_; p.P = P_;
+    p.w = w_[cur_];
+    p.one_over_dx = real(1) / dx_;
+    p.one_over_dy = real(1) / dy_;
+    p.out = static_cast<real *>(mem);
+    std::vector<real> h(3 * N);
+    hipError_t e = launch_normals(p, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), p.out, 3 * N * sizeof(real), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(mem);
+    check(e, "normals");
+    for (size_t i = 0; i < 3 * N; ++i) normal3[i] = (float)h[i];
+}
+
+void ShallowWaterEngine::addSolitaryWave(float H, float theta_deg, float x0, float y0)
+{
+    SolitonParams p;
+    p.X = X_; p.Y = Y_; p.P = P_; p.PT = PT_;
+    p.w = w_[cur_]; p.hu = hu_[cur_]; p.hvT = hvT_[cur_];
+    p.w_lo = w_lo_[cur_]; p.hu_lo = hu_lo_[cur_]; p.hvT_lo = hvT_lo_[cur_];
+    p.BA = BA_;
+    p.g = (real)s_.gravity;
+    p.dx = dx_; p.dy = dy_;
+    p.H = (real)H;
+    // sw_theta = theta * PI / 180.0f with PI = 4 atan(1) in FP32 (engine.cpp:90, 2167)
+    const float PI_F = 4.0f * std::atan(1.0f);
+    const real theta = (real)(theta_deg * PI_F / 180.0f);
+    p.cos_t = std::cos(theta);
+    p.sin_t = std::sin(theta);
+    p.x0 = (real)x0; p.y0 = (real)y0;
+    check(launch_soliton(p, stream_), "add_soliton");
 }
 
 void ShallowWaterEngine::coefficients(float *ax, float *ay)

@@ -39,6 +39,11 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
 hipError_t launch_export_coefficients(const StaticParams &p, real *ax, real *ay, hipStream_t st);
 hipError_t launch_stats(const StatsParams &p, hipStream_t st);
+hipError_t launch_gather(const real *w, const real *hu, const real *hvT, int P, int PT, const int *xy, int n,
+                         real *out, hipStream_t st);
+hipError_t launch_aux(const AuxParams &p, hipStream_t st);
+hipError_t launch_normals(const NormalParams &p, hipStream_t st);
+hipError_t launch_soliton(const SolitonParams &p, hipStream_t st);
 
 // st_getCountOfMatrices, engine.cpp:741-748
 int count_of_matrices(int n);
@@ -68,6 +73,8 @@ struct StepLaunch {
     LineParams lp;
     IrregularAll irr;
     BoundaryParams bp;
+    bool aux;                       // visual outputs on
+    AuxParams ap;
     real clock[kClockStride];
 };
 
@@ -109,6 +116,12 @@ public:
     void autotune(int repeats, cel_tuning *out) override;
     void getTuning(cel_tuning *out) const override;
     void setTuning(const cel_tuning &t) override;
+    void sample(const int *xy, int n, float *out3) override;
+    int timestepCount() const override { return timestep_count_; }
+    void setVisual(bool on, float threshold, float decay) override;
+    void getAux(float *aux3) override;
+    void getNormals(float *normal3) override;
+    void addSolitaryWave(float H, float theta_deg, float x0, float y0) override;
 
 private:
     void allocate();
@@ -172,6 +185,13 @@ private:
     float target_cfl_ = 0.f;
     void runStats(double *out9);
     void deriveStats(const double *raw, cel_stats *out) const;
+
+    // visual outputs: auxiliary planes (max depth, slope angle, whitewater), own allocation
+    bool aux_on_ = false;
+    real *aux_mem_ = nullptr;
+    real aux_threshold_ = 0, aux_rate_ = 0;      // threshold, whiteWaterDecayRate
+    void fillAux(AuxParams &p) const;
+    void zeroAux();
 
     // explicit band rows and line-solve shapes (indices into kShapeC / kShapeW)
     int rows_ = kDefaultBandRows, shape_x_ = 0, shape_y_ = 0;

@@ -41,6 +41,12 @@ public:
     virtual void autotune(int repeats, cel_tuning *out) = 0;
     virtual void getTuning(cel_tuning *out) const = 0;
     virtual void setTuning(const cel_tuning &t) = 0;
+    virtual void sample(const int *xy, int n, float *out3) = 0;
+    virtual int timestepCount() const = 0;
+    virtual void setVisual(bool on, float threshold, float decay) = 0;
+    virtual void getAux(float *aux3) = 0;
+    virtual void getNormals(float *normal3) = 0;
+    virtual void addSolitaryWave(float H, float theta_deg, float x0, float y0) = 0;
 };
 
 }  // namespace cel

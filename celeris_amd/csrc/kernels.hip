@@ -1163,6 +1163,117 @@ __global__ __launch_bounds__(64) void stats_final(const StatsParams p)
 
 // ------------------------------------------------------------------ utilities
 
+// (w, hu, hv) of n cells, xy = (column, row) pairs
+__global__ void gather_cells(const real *w, const real *hu, const real *hvT, int P, int PT, const int *xy, int n,
+                             real *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int x = xy[2 * k], y = xy[2 * k + 1];
+    out[3 * k + 0] = w[(size_t)y * P + x];
+    out[3 * k + 1] = hu[(size_t)y * P + x];
+    out[3 * k + 2] = hvT[(size_t)x * PT + y];
+}
+
+// ------------------------------------------------------------------ visual outputs
+
+// Standard deviation output of Reconstruct, compute.hlsl:306-312: of the three slope
+// estimates about half their mean.
+__device__ __forceinline__ real slope_deviation(real two_theta, real west, real here, real east)
+{
+    const real z1 = two_theta * (here - west);
+    const real z2 = (east - west);
+    const real z3 = two_theta * (east - here);
+    const real mu = real(0.5) * (z1 + z2 + z3) / real(3);
+    return SQRT(((z1 - mu) * (z1 - mu) + (z2 - mu) * (z2 - mu) + (z3 - mu) * (z3 - mu)) / real(3));
+}
+
+__device__ __forceinline__ float ATAN(float x) { return atanf(x); }
+__device__ __forceinline__ double ATAN(double x) { return atan(x); }
+__device__ __forceinline__ float COSH(float x) { return coshf(x); }
+__device__ __forceinline__ double COSH(double x) { return cosh(x); }
+
+// One step of the auxiliary texture, on Pass2's cells [1, nx+2) x [1, ny+2): Pass1 of the
+// step's starting state gives aux2 = (max(mean corrected face depth, aux1.r), 0,
+// atan(max slope deviation / dx), whitewater set to 1 above the threshold)
+// (compute.hlsl:404-457); Pass2 gives aux1 = (aux2.r, 0, aux2.b, aux2.a |decay|^dt)
+// (compute.hlsl:535-540).
+__global__ __launch_bounds__(256) void aux_update(const AuxParams p)
+{
+    const int x = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= p.nx + 2 || y >= p.ny + 2) return;
+    const size_t i = (size_t)y * p.P + x;
+    const real here = p.w[i], west = p.w[i - 1], east = p.w[i + 1];
+    const real south = p.w[i - p.P], north = p.w[i + p.P];
+    const real BN = p.BY[i], BE = p.BX[i], BS = p.BY[i - p.P], BW = p.BX[i - 1];
+    const real sx = half_slope(p.two_theta, west, here, east);
+    const real sy = half_slope(p.two_theta, south, here, north);
+    real wE = here + sx, wW = here - sx, wN = here + sy, wS = here - sy;
+    correct_w(BW, BE, here, wW, wE);
+    correct_w(BS, BN, here, wS, wN);
+    const real hN = wN - BN, hE = wE - BE, hS = wS - BS, hW = wW - BW;
+    real sd = slope_deviation(p.two_theta, west, here, east) * p.one_over_dx;
+    sd = ATAN(fmaxr(sd, slope_deviation(p.two_theta, south, here, north) * p.one_over_dy));
+    p.depth[i] = fmaxr((hN + hE + hS + hW) / real(4), p.depth[i]);
+    p.slope[i] = sd;
+    const real white = sd > p.threshold ? real(1) : p.white[i];
+    p.white[i] = white * p.decay;
+}
+
+// The normal texture of Pass1 (compute.hlsl:437-445) on [1, X-1) x [1, Y-1): normalize of
+// ((w_W - w_E)/dx, (w_S - w_N)/dy, 2), taken as v / |v|.
+__global__ __launch_bounds__(256) void state_normals(const NormalParams p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.X * p.Y) return;
+    const int x = k % p.X, y = k / p.X;
+    const size_t N = (size_t)p.X * p.Y;
+    real n0 = 0, n1 = 0, n2 = 0;
+    if (x >= 1 && x < p.X - 1 && y >= 1 && y < p.Y - 1) {
+        const size_t i = (size_t)y * p.P + x;
+        const real a = (p.w[i - 1] - p.w[i + 1]) * p.one_over_dx;
+        const real b = (p.w[i - p.P] - p.w[i + p.P]) * p.one_over_dy;
+        const real c = real(2);
+        const real len = SQRT(a * a + b * b + c * c);
+        n0 = a / len; n1 = b / len; n2 = c / len;
+    }
+    p.out[k] = n0; p.out[N + k] = n1; p.out[2 * N + k] = n2;
+}
+
+// addSolitaryWave + solitaryWave (engine.cpp:2154-2220, compute.hlsl:947-976): every cell of
+// the ghost-inclusive grid at x = i dx, y = j dy, local depth d = max(w - BA, 0):
+// eta = H / cosh(K ((x - x0) cos + (y - y0) sin))^2, K = sqrt(0.75 |H| / d^3),
+// C = sqrt(g (H + d)); (w, hu, hv) += (eta, C cos eta, C sin eta).  A compensated state
+// takes the increment into its low words with an exact two-sum.
+__global__ __launch_bounds__(256) void add_soliton(const SolitonParams p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.X * p.Y) return;
+    const int x = k % p.X, y = k / p.X;
+    const size_t i = (size_t)y * p.P + x, t = (size_t)x * p.PT + y;
+    const real w = p.w[i];
+    const real d = fmaxr(w - p.BA[i], real(0));
+    const real K = SQRT(real(0.75) * FABS(p.H) / (d * d * d));
+    const real C = SQRT(p.g * (p.H + d));
+    const real ch = COSH(K * (((real)x * p.dx - p.x0) * p.cos_t + ((real)y * p.dy - p.y0) * p.sin_t));
+    const real eta = p.H * real(1) / (ch * ch);
+    const real dhu = C * p.cos_t * eta, dhv = C * p.sin_t * eta;
+    if (p.w_lo) {
+        real s, e;
+        two_sum(w, eta, s, e);
+        p.w[i] = s; p.w_lo[i] += e;
+        two_sum(p.hu[i], dhu, s, e);
+        p.hu[i] = s; p.hu_lo[i] += e;
+        two_sum(p.hvT[t], dhv, s, e);
+        p.hvT[t] = s; p.hvT_lo[t] += e;
+    } else {
+        p.w[i] = w + eta;
+        p.hu[i] = p.hu[i] + dhu;
+        p.hvT[t] = p.hvT[t] + dhv;
+    }
+}
+
 // out[j * pout + i] = in[i * pin + j] for i < rows, j < cols (host-side layout changes)
 __global__ void transpose_plane(const real *in, int pin, real *out, int pout, int rows, int cols)
 {
@@ -1375,6 +1486,33 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st)
 {
     hipLaunchKernelGGL(static_fields, dim3((p.nx + 127) / 128, p.ny), dim3(128), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const real *w, const real *hu, const real *hvT, int P, int PT, const int *xy, int n,
+                         real *out, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(gather_cells, dim3((n + 255) / 256), dim3(256), 0, st, w, hu, hvT, P, PT, xy, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_aux(const AuxParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(aux_update, dim3((p.nx + 1 + 63) / 64, (p.ny + 1 + 3) / 4), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(const NormalParams &p, hipStream_t st)
+{
+    const int n = p.X * p.Y;
+    hipLaunchKernelGGL(state_normals, dim3((n + 255) / 256), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_soliton(const SolitonParams &p, hipStream_t st)
+{
+    const int n = p.X * p.Y;
+    hipLaunchKernelGGL(add_soliton, dim3((n + 255) / 256), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 

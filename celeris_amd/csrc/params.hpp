@@ -176,6 +176,34 @@ struct StatsParams {
     double *out;       // kStatsValues
 };
 
+// Visual outputs of one step (Pass1 aux + Pass2 decay): the state at the start of the
+// step, its face bottoms, and the three auxiliary planes updated in place.
+struct AuxParams {
+    int nx, ny, P;
+    const real *w, *BX, *BY;
+    real *depth, *slope, *white;
+    real two_theta, one_over_dx, one_over_dy;
+    real threshold;    // dissipation threshold on the slope angle
+    real decay;        // |whiteWaterDecayRate|^dt of the step
+};
+
+// addSolitaryWave over the whole ghost-inclusive grid.
+struct SolitonParams {
+    int X, Y, P, PT;
+    real *w, *hu, *hvT;
+    real *w_lo, *hu_lo, *hvT_lo;     // compensated state, else null
+    const real *BA;
+    real g, dx, dy, H, cos_t, sin_t, x0, y0;
+};
+
+// Normals of the current state (Pass1's visual target), planes of X x Y.
+struct NormalParams {
+    int X, Y, P;
+    const real *w;
+    real one_over_dx, one_over_dy;
+    real *out;         // 3 planes, row-major X x Y
+};
+
 }  // namespace CEL_NS
 
 #endif

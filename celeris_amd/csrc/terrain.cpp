@@ -43,8 +43,10 @@ void solitary_wave(float g, float depth, float H, float theta, float x0, float y
 }
 }  // namespace
 
-// createSimTextures initial condition without a hot-start file, engine.cpp:2945-3016.
-void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3)
+// createSimTextures initial condition, engine.cpp:2945-3016: w = max(SWL, B), hu = hv = 0
+// (or the hot-start values base3 when given), plus every solitary wave.
+void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3,
+                   const float *base3)
 {
     const int nx = s.nx, ny = s.ny, X = nx + 4, Y = ny + 4;
     const size_t N = (size_t)X * Y;
@@ -58,7 +60,9 @@ void initial_state(const cel_settings &s, const float *bottom3, const cel_solito
             const int jj = std::max(2, std::min(ny + 1, j));
             const float B = BA[(size_t)X * jj + ii];
             const float waterDepth = std::max(0.0f, s.still_water_elevation - B);
-            float w = std::max(s.still_water_elevation, B), hu = 0.f, hv = 0.f;
+            const size_t c0 = (size_t)j * X + i;
+            float w = base3 ? base3[c0] : std::max(s.still_water_elevation, B);
+            float hu = base3 ? base3[N + c0] : 0.f, hv = base3 ? base3[2 * N + c0] : 0.f;
             for (int k = 0; k < nsol; ++k) {
                 const cel_soliton &t = sol[k];
                 if (t.param_h == 0) continue;

@@ -108,6 +108,12 @@ typedef struct {
 
 typedef struct cel_engine cel_engine;
 
+/* The same initial condition starting from a hot-start state (engine.cpp:2943-2970): w,
+ * hu, hv from hot3 (the file's values, ghost-inclusive planes), then the solitary waves
+ * added on top. */
+int cel_hot_start_state(const cel_settings *s, const float *bottom3, const float *hot3, const cel_soliton *solitons,
+                        int n_solitons, float *state3);
+
 /* Thread-local description of the last failed call (empty if none). */
 const char *cel_last_error(void);
 
@@ -156,6 +162,10 @@ int cel_engine_sync(cel_engine *e);
 int cel_engine_get_state(cel_engine *e, float *state3);
 /* The state as FP64 planes: hi + lo for a compensated engine, hi otherwise. */
 int cel_engine_get_state_f64(cel_engine *e, double *state3);
+/* (w, hu, hv) at n cells (xy[2k], xy[2k+1] = ghost-inclusive column and row) of the
+ * current state, out3[3k..3k+2]; synchronises.  The gauge reads of the logger
+ * (st_DumpToFile, engine.cpp:628-675) without copying the whole state. */
+int cel_engine_sample(cel_engine *e, const int *xy, int n, float *out3);
 /* Hot start (engine.cpp:2943-2970): replaces the state; the AB history and the
  * scheme counters are reset exactly as after remesh. */
 int cel_engine_set_state(cel_engine *e, const float *state3);
@@ -163,6 +173,32 @@ int cel_engine_set_state(cel_engine *e, const float *state3);
 /* Virtual time total_time (engine.cpp:1518) and the scheme of the next step. */
 int cel_engine_time(cel_engine *e, double *t);
 int cel_engine_scheme(cel_engine *e, int *scheme);
+/* The reference's timestep_count (engine.cpp:1491-1495): 1 after creation or a hot start,
+ * incremented by every step after which the next scheme is Euler or a predictor.  The
+ * logger's schedule (a record when it is a multiple of logStep) is keyed on it. */
+int cel_engine_timestep_count(cel_engine *e, int *count);
+
+/* Visual outputs of the step (compute.hlsl:437-457, 535-540).  When enabled, every step
+ * updates the auxiliary texture m_psAuxiliary1Texture from the step's starting state on
+ * the cells [1, nx+2) x [1, ny+2): r = maximum over steps of the mean corrected face
+ * depth (the inundation record dumped by dumpInundationToFile, engine.cpp:781-817),
+ * b = atan of the largest w-slope deviation per metre, a = whitewater (1 where b exceeds
+ * dissipation_threshold, decayed by (1 / (1 + 2 whitewater_decay))^dt every step).  The
+ * reference defaults are 0.25 and 0.1 (main.cpp:971-977).  Enabling (or a hot start)
+ * starts the record from zero; disabled engines do no extra work. */
+int cel_engine_set_visual(cel_engine *e, int enable, float dissipation_threshold, float whitewater_decay);
+/* The auxiliary planes (max inundated depth, slope angle, whitewater), each
+ * (nx+4) x (ny+4) ghost-inclusive; CEL_EINVAL when visual outputs are off. */
+int cel_engine_get_aux(cel_engine *e, float *aux3);
+/* The normal texture of the current state (Pass1, compute.hlsl:437-445): planes nx, ny,
+ * nz of normalize((w_W - w_E)/dx, (w_S - w_N)/dy, 2) on [1, nx+3) x [1, ny+3), 0 outside. */
+int cel_engine_get_normals(cel_engine *e, float *normal3);
+/* addSolitaryWave (engine.cpp:2154-2220, compute.hlsl:947-976): adds the solitary wave
+ * (H, direction theta_deg in degrees, crest through (x0, y0) m) to the current state over
+ * the whole ghost-inclusive grid, with the local depth max(w - BA, 0).  The next step's
+ * fluxes see the new state (the reference's next Pass2 still reads the H, U, V of the
+ * Pass1 that ran before the wave was added). */
+int cel_engine_add_solitary_wave(cel_engine *e, float H, float theta_deg, float x0, float y0);
 
 /* Runs nsteps and returns the elapsed device time in ms, measured with HIP events
  * on the engine's stream (inputs are already resident in HBM). */

@@ -58,6 +58,11 @@ def lib():
             getattr(L, f"orc_{p}_stats").argtypes = [ctypes.c_void_p, _D]
             getattr(L, f"orc_{p}_update_timestep").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4 + [_D]
             getattr(L, f"orc_{p}_update_timestep").restype = ctypes.c_float
+            getattr(L, f"orc_{p}_set_visual").argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+            getattr(L, f"orc_{p}_get_aux").argtypes = [ctypes.c_void_p, R]
+            getattr(L, f"orc_{p}_get_normals").argtypes = [ctypes.c_void_p, R]
+            getattr(L, f"orc_{p}_add_solitary_wave").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4 + [ctypes.c_int]
+            getattr(L, f"orc_{p}_refresh").argtypes = [ctypes.c_void_p]
         L.orc_bottom_from_corners.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
         L.orc_f32_minmod.restype = ctypes.c_float
         L.orc_f32_minmod.argtypes = [ctypes.c_float] * 3
@@ -152,6 +157,29 @@ class Oracle:
         u = self._uniform_dt if uniform_dt is None else uniform_dt
         dt = getattr(lib(), f"orc_{self.p}_update_timestep")(self.h, ema_alpha, min_ratio, target_cfl, u, dptr(out))
         return dt, out
+
+    def set_visual(self, dissipation_threshold=0.25, whitewater_decay=0.1):
+        getattr(lib(), f"orc_{self.p}_set_visual")(self.h, dissipation_threshold, whitewater_decay)
+
+    def _planes(self, name):
+        dt = np.float32 if self.p == "f32" else np.float64
+        out = np.zeros((3, self.ny + 4, self.nx + 4), dt)
+        getattr(lib(), f"orc_{self.p}_{name}")(self.h, fptr(out) if self.p == "f32" else dptr(out))
+        return out
+
+    def aux(self):
+        """(max inundated depth, slope angle, whitewater) planes of the auxiliary texture."""
+        return self._planes("get_aux")
+
+    def normals(self):
+        return self._planes("get_normals")
+
+    def refresh(self):
+        """Pass1 of the current state (H, U, V and the normal), as the bootstrap Pass1."""
+        getattr(lib(), f"orc_{self.p}_refresh")(self.h)
+
+    def add_solitary_wave(self, H, theta_deg, x0, y0, refresh=False):
+        getattr(lib(), f"orc_{self.p}_add_solitary_wave")(self.h, H, theta_deg, x0, y0, int(refresh))
 
     def coefficients(self):
         dt = np.float32 if self.p == "f32" else np.float64

@@ -83,6 +83,18 @@ void orc_f32_stats(const orc_sim_f32 *s, double *out9);
 float orc_f32_update_timestep(orc_sim_f32 *s, float ema_alpha, float min_ratio, float target_cfl, float uniform_dt,
                               double *stats9);
 
+/* Visual outputs of Pass1/Pass2 (compute.hlsl:437-457, 535-540).  set_visual sets the
+ * dissipation threshold and the "Whitewater Decay" setting (defaults 0.25 and 0.1, the
+ * CML defaults); get_aux returns m_psAuxiliary1Texture as three planes (max inundated
+ * depth, slope angle, whitewater), get_normals the normal texture (nx, ny, nz). */
+void orc_f32_set_visual(orc_sim_f32 *s, float dissipation_threshold, float whitewater_decay);
+void orc_f32_get_aux(const orc_sim_f32 *s, float *out3);
+void orc_f32_get_normals(const orc_sim_f32 *s, float *out3);
+/* addSolitaryWave (engine.cpp:2154-2220); refresh != 0 re-runs Pass1 on the new state. */
+void orc_f32_add_solitary_wave(orc_sim_f32 *s, float H, float theta_deg, float x0, float y0, int refresh);
+/* Pass1 of the current state (as the bootstrap Pass1), refreshing H, U, V and the normal. */
+void orc_f32_refresh(orc_sim_f32 *s);
+
 orc_sim_f64 *orc_f64_create(const orc_config *cfg, const float *bottom, const float *state);
 void orc_f64_destroy(orc_sim_f64 *s);
 void orc_f64_step(orc_sim_f64 *s, int nsteps);
@@ -94,6 +106,11 @@ void orc_f64_coefficients(const orc_sim_f64 *s, double *abc_x3, double *abc_y3);
 void orc_f64_stats(const orc_sim_f64 *s, double *out9);
 float orc_f64_update_timestep(orc_sim_f64 *s, float ema_alpha, float min_ratio, float target_cfl, float uniform_dt,
                               double *stats9);
+void orc_f64_set_visual(orc_sim_f64 *s, float dissipation_threshold, float whitewater_decay);
+void orc_f64_get_aux(const orc_sim_f64 *s, double *out3);
+void orc_f64_get_normals(const orc_sim_f64 *s, double *out3);
+void orc_f64_add_solitary_wave(orc_sim_f64 *s, float H, float theta_deg, float x0, float y0, int refresh);
+void orc_f64_refresh(orc_sim_f64 *s);
 
 /* Terrain preprocessing, terrain_heightfield.cpp:244-298: corner heights
  * z[(ny+5)*(nx+5)] -> bottom planes BY, BX, BA. */

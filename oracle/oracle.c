@@ -19,6 +19,7 @@ typedef float real;
 #define POW powf
 #define FABS fabsf
 #define ATAN atanf
+#define COSH coshf
 #include "oracle_impl.inc"
 #undef FN
 #undef SIM
@@ -29,6 +30,7 @@ typedef float real;
 #undef POW
 #undef FABS
 #undef ATAN
+#undef COSH
 #undef NUM_TEX
 #undef MAXLEV
 #undef CURRENT_STATE
@@ -43,6 +45,7 @@ typedef float real;
 #define POW pow
 #define FABS fabs
 #define ATAN atan
+#define COSH cosh
 #include "oracle_impl.inc"
 #undef real
 
