@@ -148,3 +148,29 @@ def test_conical_island_runs_like_the_oracle(tmp_path):
     txt = open(tmp_path / lg.dump_inundation()).read()
     assert txt.startswith("[nx] 201\n[ny] 201\n")
     e.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cml,steps", [("flume.cml", 300), ("diagSolitary.cml", 150), ("irregular.cml", 150),
+                                       ("uniform_time_series.cml", 150)])
+def test_reference_examples_run_like_the_oracle(cml, steps):
+    """The reference's other example experiments through the drop-in (sponge flume with a
+    soliton, four diagonal solitons in a walled basin, irregular waves and uniform time
+    series on all four sides of the conical island): within 1e-5 of the FP32 oracle with
+    the same wet/dry mask."""
+    from oracle.binding import Oracle
+    from tests.parity import eta, rel_err, wet_mask
+    ex = I.load_experiment(os.path.join(EX, cml))
+    e = ShallowWaterEngine(ex.s, ex.boundaries, ex.bottom, ex.state)
+    o = Oracle(ex.s, ex.boundaries, ex.bottom, ex.state, "f32")
+    e.timestep(steps)
+    o.step(steps)
+    assert e.time == o.time
+    a, b = e.get_state(), o.state()
+    e.close()
+    assert np.isfinite(a).all()
+    swl = ex.s.still_water_elevation
+    err = rel_err(eta(a, swl), eta(b, swl))
+    assert max(err.values()) < 1e-5, err
+    assert (wet_mask(a, ex.bottom) == wet_mask(b, ex.bottom)).all()
+    assert np.abs(b[1:]).max() > 1e-3                   # the forcing or the waves moved water
