@@ -77,6 +77,10 @@ SIGNATURES = {
     "cel_engine_get_aux": ([ctypes.c_void_p, _F], ctypes.c_int),
     "cel_engine_get_normals": ([ctypes.c_void_p, _F], ctypes.c_int),
     "cel_engine_add_solitary_wave": ([ctypes.c_void_p] + [ctypes.c_float] * 4, ctypes.c_int),
+    "cel_engine_set_bottom": ([ctypes.c_void_p, _F, ctypes.c_int], ctypes.c_int),
+    "cel_engine_raise_terrain": ([ctypes.c_void_p] + [ctypes.c_float] * 4, ctypes.c_int),
+    "cel_engine_shift_terrain": ([ctypes.c_void_p, ctypes.c_float], ctypes.c_int),
+    "cel_engine_edit_water": ([ctypes.c_void_p] + [ctypes.c_float] * 5, ctypes.c_int),
     "cel_engine_create": ([ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
                            ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "cel_engine_destroy": ([ctypes.c_void_p], ctypes.c_int),
@@ -316,6 +320,21 @@ class ShallowWaterEngine:
 
     def add_solitary_wave(self, H, theta_deg, x0, y0):
         _check(load_library().cel_engine_add_solitary_wave(self._h, H, theta_deg, x0, y0))
+
+    def set_bottom(self, bottom, keep_depth=True):
+        """New bottom planes (BY, BX, BA); keep_depth moves the water with the terrain."""
+        bottom = np.ascontiguousarray(bottom, dtype=np.float32)
+        _check(load_library().cel_engine_set_bottom(self._h, _fptr(bottom), int(keep_depth)))
+
+    def raise_terrain(self, x, y, radius, displacement):
+        _check(load_library().cel_engine_raise_terrain(self._h, x, y, radius, displacement))
+
+    def shift_terrain(self, shift_up):
+        _check(load_library().cel_engine_shift_terrain(self._h, shift_up))
+
+    def edit_water(self, x, y, radius, disp_A, disp_B=0.0):
+        """Left-mouse brush: add water (s, 0), remove (-s, 0), stir (-s, 1.5 s)."""
+        _check(load_library().cel_engine_edit_water(self._h, x, y, radius, disp_A, disp_B))
 
     def timed_steps(self, n):
         ms = ctypes.c_float()

@@ -129,6 +129,26 @@ int cel_engine_add_solitary_wave(cel_engine *e, float H, float theta_deg, float 
     return guarded([&] { eng(e).addSolitaryWave(H, theta_deg, x0, y0); });
 }
 
+int cel_engine_set_bottom(cel_engine *e, const float *bottom3, int keep_depth)
+{
+    return guarded([&] { eng(e).setBottom(bottom3, keep_depth != 0); });
+}
+
+int cel_engine_raise_terrain(cel_engine *e, float x, float y, float radius, float displacement)
+{
+    return guarded([&] { eng(e).raiseTerrain(x, y, radius, displacement); });
+}
+
+int cel_engine_shift_terrain(cel_engine *e, float shift_up)
+{
+    return guarded([&] { eng(e).shiftTerrain(shift_up); });
+}
+
+int cel_engine_edit_water(cel_engine *e, float x, float y, float radius, float disp_A, float disp_B)
+{
+    return guarded([&] { eng(e).editWater(x, y, radius, disp_A, disp_B); });
+}
+
 int cel_engine_create(const cel_settings *s, const cel_boundary bnd[4], const float *bottom3, const float *state3,
                       cel_engine **out)
 {

@@ -177,8 +177,16 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
 void ShallowWaterEngine::uploadBottom(const float *bottom3)
 {
     const size_t N = (size_t)X_ * Y_;
+    hostBottom_.assign(bottom3, bottom3 + 3 * N);
+    uploadHostBottom();
+}
+
+// Uploads hostBottom_ (BY, BX, BA planes) and recomputes the static per-cell fields.
+void ShallowWaterEngine::uploadHostBottom()
+{
+    const size_t N = (size_t)X_ * Y_;
     const size_t e = sizeof(real);
-    const std::vector<real> b(bottom3, bottom3 + 3 * N);
+    const std::vector<real> &b = hostBottom_;
     real *dst[3] = {BY_, BX_, BA_};
     for (int c = 0; c < 3; ++c)
         check(hipMemcpy2DAsync(dst[c], (size_t)P_ * e, b.data() + c * N, (size_t)X_ * e, (size_t)X_ * e, Y_,
@@ -1110,6 +1118,159 @@ void ShallowWaterEngine::sample(const int *xy, int n, float *out3)
     (void)hipFree(mem);
     check(e, "sample");
     for (size_t i = 0; i < h.size(); ++i) out3[i] = (float)h[i];
+}
+
+// ---- terrain and water edits
+
+// The w plane of the current state on the host (ghost-inclusive, row-major) and back.
+void ShallowWaterEngine::downloadW(std::vector<real> &w)
+{
+    const size_t e = sizeof(real);
+    w.resize((size_t)X_ * Y_);
+    check(hipMemcpy2DAsync(w.data(), (size_t)X_ * e, w_[cur_], (size_t)P_ * e, (size_t)X_ * e, Y_,
+                           hipMemcpyDeviceToHost, stream_), "download w");
+    check(hipStreamSynchronize(stream_), "sync");
+}
+
+void ShallowWaterEngine::uploadW(const std::vector<real> &w)
+{
+    const size_t e = sizeof(real);
+    check(hipMemcpy2DAsync(w_[cur_], (size_t)P_ * e, w.data(), (size_t)X_ * e, (size_t)X_ * e, Y_,
+                           hipMemcpyHostToDevice, stream_), "upload w");
+}
+
+// After a bottom change: static fields, the depth-dependent boundary tables, no captured
+// blocks (their tables moved).
+void ShallowWaterEngine::bottomChanged()
+{
+    clearGraphs();
+    uploadHostBottom();
+    for (int i = 0; i < 4; ++i) buildSideTables(i);
+}
+
+// fillTerrainTexture (keep_depth) / fillTerrainTextureLite, engine.cpp:3313-3412.
+void ShallowWaterEngine::setBottom(const float *bottom3, bool keep_depth)
+{
+    if (!bottom3) throw Error(CEL_EINVAL, "null bottom");
+    sync();
+    const size_t N = (size_t)X_ * Y_;
+    if (keep_depth) {
+        std::vector<real> w;
+        downloadW(w);
+        for (size_t i = 0; i < N; ++i) w[i] = (w[i] - hostBottom_[2 * N + i]) + (real)bottom3[2 * N + i];
+        uploadW(w);
+    }
+    hostBottom_.assign(bottom3, bottom3 + 3 * N);
+    if (aux_on_) zeroAux();
+    bottomChanged();
+}
+
+namespace {
+// GaussianBrush, engine.cpp:93-99
+real gaussian_brush(real x, real y, real r)
+{
+    const real sigma = real(1.0f) / real(3.0f);
+    const real z = (x * x + y * y) / (r * r);
+    if (z > real(1)) return real(0);
+    return real(2) * std::exp(-z / (real(2) * sigma * sigma));
+}
+}  // namespace
+
+// raiseLowerTerrain, engine.cpp:4685-4843 (CPU code in the reference too).
+void ShallowWaterEngine::raiseTerrain(float world_x, float world_y, float radius, float displacement)
+{
+    sync();
+    const size_t N = (size_t)X_ * Y_;
+    const float W = s_.width, L = s_.length;
+    const float brush_radius = radius * 2;
+    const float sx = (nx_ - 1) / W, bx = (nx_ + 4) / 2.0f, sy = (ny_ - 1) / L, by = 2.5f;
+    const int cix = (int)(sx * world_x + bx), ciy = (int)(sy * world_y + by);
+    const int rx = (int)(sx * brush_radius) + 3, ry = (int)(sy * brush_radius) + 3;
+    const int ix0 = std::max(0, std::min(nx_ + 4, cix - rx)), ix1 = std::max(0, std::min(nx_ + 4, cix + rx + 1));
+    const int iy0 = std::max(0, std::min(ny_ + 4, ciy - ry)), iy1 = std::max(0, std::min(ny_ + 4, ciy + ry + 1));
+    std::vector<real> w;
+    downloadW(w);
+    real *BY = hostBottom_.data(), *BX = BY + N, *BA = BY + 2 * N;
+    const real br = (real)brush_radius, disp = (real)displacement;
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const real wyp = (real)((iy + 0.5f - by) / sy - world_y), wym = (real)((iy - 0.5f - by) / sy - world_y);
+        for (int ix = ix0; ix < ix1; ++ix) {
+            const real wxp = (real)((ix + 0.5f - bx) / sx - world_x), wxm = (real)((ix - 0.5f - bx) / sx - world_x);
+            const real pp = gaussian_brush(wxp, wyp, br) * disp, pm = gaussian_brush(wxp, wym, br) * disp;
+            const real mp = gaussian_brush(wxm, wyp, br) * disp, mm = gaussian_brush(wxm, wym, br) * disp;
+            const size_t i = (size_t)iy * X_ + ix;
+            w[i] += (pp + pm + mp + mm) / real(4);
+            BX[i] += (pp + pm) / real(2);
+            BY[i] += (pp + mp) / real(2);
+            BA[i] += (pp + pm + mp + mm) / real(4);
+        }
+    }
+    uploadW(w);
+    bottomChanged();
+}
+
+// shiftTerrainSlider (engine.cpp:4845-4924) with the auxiliary reset resetTimestep does after
+// it (engine.cpp:1544-1553).
+void ShallowWaterEngine::shiftTerrain(float shift_up)
+{
+    sync();
+    const size_t N = (size_t)X_ * Y_;
+    std::vector<real> w;
+    downloadW(w);
+    real *BY = hostBottom_.data(), *BX = BY + N, *BA = BY + 2 * N;
+    const real swl = (real)s_.still_water_elevation, sh = (real)shift_up;
+    for (size_t i = 0; i < N; ++i) {
+        if ((double)(w[i] - BA[i]) < 0.01) {
+            if (BA[i] < swl) w[i] = swl;
+            else w[i] += sh;
+        }
+        BX[i] += sh;
+        BY[i] += sh;
+        BA[i] += sh;
+    }
+    uploadW(w);
+    if (aux_on_) zeroAux();
+    bottomChanged();
+}
+
+// The left-mouse water brush: 12-triangle fan rasterised with D3D11's rules on the cells'
+// centres (applyMouseShader, engine.cpp:4561-4683; left_mouse.hlsl:59-103).
+void ShallowWaterEngine::editWater(float world_x, float world_y, float radius, float disp_A, float disp_B)
+{
+    if (!(radius > 0.f)) throw Error(CEL_EINVAL, "brush radius must be positive");
+    WaterBrushParams p;
+    const float W = s_.width, L = s_.length;
+    const float sxw = (nx_ - 1) / W, syw = (ny_ - 1) / L;
+    const float scx = sxw * radius, bcx = sxw * world_x + (nx_ + 4) / 2.0f;
+    const float scy = syw * radius, bcy = syw * world_y + 2.5f;
+    const float PI_F = 4.0f * std::atan(1.0f), dth = 2 * PI_F / 12;
+    double lo_x = 1e300, hi_x = -1e300, lo_y = 1e300, hi_y = -1e300;
+    for (int k = 0; k <= 12; ++k) {
+        float px = 0.f, py = 0.f;
+        if (k < 12) { const float th = dth * k; px = std::cos(th); py = std::sin(th); }
+        const float tx = scx * px + bcx, ty = scy * py + bcy;
+        // vertex positions snapped to 1/256 of a cell (k = 12: the centre)
+        p.vx[k] = std::nearbyint((double)tx * 256.0) / 256.0;
+        p.vy[k] = std::nearbyint((double)ty * 256.0) / 256.0;
+        lo_x = std::min(lo_x, p.vx[k]); hi_x = std::max(hi_x, p.vx[k]);
+        lo_y = std::min(lo_y, p.vy[k]); hi_y = std::max(hi_y, p.vy[k]);
+    }
+    if (!(hi_x > lo_x) || !(hi_y > lo_y) || !std::isfinite(hi_x + hi_y + lo_x + lo_y)) return;
+    // cells whose centres can lie inside the fan, clipped to the grid
+    p.x0 = (int)std::max(0.0, std::floor(lo_x - 1.0));
+    p.y0 = (int)std::max(0.0, std::floor(lo_y - 1.0));
+    const int x1 = (int)std::min((double)X_, std::ceil(hi_x + 1.0));
+    const int y1 = (int)std::min((double)Y_, std::ceil(hi_y + 1.0));
+    if (x1 <= p.x0 || y1 <= p.y0) return;
+    p.bw = x1 - p.x0;
+    p.bh = y1 - p.y0;
+    p.P = P_; p.PT = PT_;
+    p.w = w_[cur_]; p.hu = hu_[cur_]; p.hvT = hvT_[cur_];
+    p.w_lo = w_lo_[cur_]; p.hu_lo = hu_lo_[cur_]; p.hvT_lo = hvT_lo_[cur_];
+    p.BA = BA_;
+    p.disp_A = disp_A;
+    p.disp_B = disp_B;
+    check(launch_water_brush(p, stream_), "water_brush");
 }
 
 // ---- visual outputs (compute.hlsl:437-457, 535-540) and addSolitaryWave

@@ -44,6 +44,7 @@ hipError_t launch_gather(const real *w, const real *hu, const real *hvT, int P, 
 hipError_t launch_aux(const AuxParams &p, hipStream_t st);
 hipError_t launch_normals(const NormalParams &p, hipStream_t st);
 hipError_t launch_soliton(const SolitonParams &p, hipStream_t st);
+hipError_t launch_water_brush(const WaterBrushParams &p, hipStream_t st);
 
 // st_getCountOfMatrices, engine.cpp:741-748
 int count_of_matrices(int n);
@@ -122,10 +123,18 @@ public:
     void getAux(float *aux3) override;
     void getNormals(float *normal3) override;
     void addSolitaryWave(float H, float theta_deg, float x0, float y0) override;
+    void setBottom(const float *bottom3, bool keep_depth) override;
+    void raiseTerrain(float world_x, float world_y, float radius, float displacement) override;
+    void shiftTerrain(float shift_up) override;
+    void editWater(float world_x, float world_y, float radius, float disp_A, float disp_B) override;
 
 private:
     void allocate();
     void uploadBottom(const float *bottom3);
+    void uploadHostBottom();
+    void bottomChanged();
+    void downloadW(std::vector<real> &w);
+    void uploadW(const std::vector<real> &w);
     void resetScheme();
     void buildSideTables(int side);
     void freeSideTables(int side);
@@ -175,6 +184,7 @@ private:
     real *Gp_[3], *FGp_[2];
     real *Cw_, *Chu_, *ChvT_;
     std::vector<real> hostBA_;
+    std::vector<real> hostBottom_;      // BY, BX, BA planes, ghost-inclusive (terrain edits)
 
     // GetStats buffers and resetTimestep's adaptive-dt state (engine.cpp:1698-1726)
     double *d_stats_ = nullptr;     // partials, then the nine results

@@ -47,6 +47,10 @@ public:
     virtual void getAux(float *aux3) = 0;
     virtual void getNormals(float *normal3) = 0;
     virtual void addSolitaryWave(float H, float theta_deg, float x0, float y0) = 0;
+    virtual void setBottom(const float *bottom3, bool keep_depth) = 0;
+    virtual void raiseTerrain(float world_x, float world_y, float radius, float displacement) = 0;
+    virtual void shiftTerrain(float shift_up) = 0;
+    virtual void editWater(float world_x, float world_y, float radius, float disp_A, float disp_B) = 0;
 };
 
 }  // namespace cel

@@ -1274,6 +1274,62 @@ __global__ __launch_bounds__(256) void add_soliton(const SolitonParams p)
     }
 }
 
+// D3D11 coverage of a cell centre by one fan triangle (a = centre, b, c on the rim):
+// clockwise on screen (y down), drawn when every edge function is positive or zero on a top
+// or left edge; lc receives the barycentric weight of the centre vertex.  Coordinates are
+// multiples of 1/256 below 2^15, so the edge functions are exact in double.
+__device__ __forceinline__ bool fan_covers(double ax, double ay, double bx, double by, double cx, double cy, double px,
+                                           double py, double &lc)
+{
+    const double xs[3] = {ax, bx, cx}, ys[3] = {ay, by, cy};
+    const double area = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+    if (area == 0.0) return false;
+    const double sg = area > 0 ? 1.0 : -1.0;
+    double e1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int k1 = (k + 1) % 3;
+        double dx = xs[k1] - xs[k], dy = ys[k1] - ys[k];
+        const double e = sg * (dx * (py - ys[k]) - dy * (px - xs[k]));
+        dx *= sg; dy *= sg;
+        const bool top_left = dy < 0 || (dy == 0 && dx > 0);
+        if (e < 0 || (e == 0 && !top_left)) return false;
+        if (k == 1) e1 = e;
+    }
+    lc = e1 / (sg * area);
+    return true;
+}
+
+// LeftMousePixelShader (left_mouse.hlsl:80-103) on the cells the brush fan covers:
+// dh = disp_A + disp_B radius, w' = max(B, w + dh), (hu, hv)' = (w' - B) (hu, hv) / max(0.5, w - B).
+// A compensated state's low words of edited cells are cleared (the values are set).
+__global__ __launch_bounds__(256) void water_brush(const WaterBrushParams p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.bw * p.bh) return;
+    const int i = p.x0 + k % p.bw, j = p.y0 + k / p.bw;
+    const double px = i + 0.5, py = j + 0.5;
+    double lc = 0.0;
+    bool hit = false;
+    for (int t = 0; t < 12 && !hit; ++t)
+        hit = fan_covers(p.vx[12], p.vy[12], p.vx[t], p.vy[t], p.vx[(t + 1) % 12], p.vy[(t + 1) % 12], px, py, lc);
+    if (!hit) return;
+    const float rad = (float)(1.0 - lc);
+    const size_t c = (size_t)j * p.P + i, ct = (size_t)i * p.PT + j;
+    const real B = p.BA[c];
+    const real w = p.w[c];
+    const real dh = (real)p.disp_A + (real)p.disp_B * (real)rad;
+    const real old_h = w - B;
+    const real new_w = fmaxr(B, w + dh);
+    const real new_h = new_w - B;
+    const real lim = fmaxr(real(0.5f), old_h);
+    const real u = p.hu[c] / lim, v = p.hvT[ct] / lim;
+    p.w[c] = new_w;
+    p.hu[c] = new_h * u;
+    p.hvT[ct] = new_h * v;
+    if (p.w_lo) { p.w_lo[c] = 0; p.hu_lo[c] = 0; p.hvT_lo[ct] = 0; }
+}
+
 // out[j * pout + i] = in[i * pin + j] for i < rows, j < cols (host-side layout changes)
 __global__ void transpose_plane(const real *in, int pin, real *out, int pout, int rows, int cols)
 {
@@ -1513,6 +1569,13 @@ hipError_t launch_soliton(const SolitonParams &p, hipStream_t st)
 {
     const int n = p.X * p.Y;
     hipLaunchKernelGGL(add_soliton, dim3((n + 255) / 256), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_water_brush(const WaterBrushParams &p, hipStream_t st)
+{
+    const int n = p.bw * p.bh;
+    if (n > 0) hipLaunchKernelGGL(water_brush, dim3((n + 255) / 256), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 

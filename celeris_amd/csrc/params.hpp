@@ -196,6 +196,17 @@ struct SolitonParams {
     real g, dx, dy, H, cos_t, sin_t, x0, y0;
 };
 
+// The left-mouse water brush: fan vertices (0..11 rim, 12 centre) in cell-index units,
+// snapped to 1/256, and the cell box [x0, x0 + bw) x [y0, y0 + bh) that can be covered.
+struct WaterBrushParams {
+    double vx[13], vy[13];
+    int x0, y0, bw, bh, P, PT;
+    real *w, *hu, *hvT;
+    real *w_lo, *hu_lo, *hvT_lo;     // compensated state, else null
+    const real *BA;
+    float disp_A, disp_B;
+};
+
 // Normals of the current state (Pass1's visual target), planes of X x Y.
 struct NormalParams {
     int X, Y, P;

@@ -200,6 +200,34 @@ int cel_engine_get_normals(cel_engine *e, float *normal3);
  * Pass1 that ran before the wave was added). */
 int cel_engine_add_solitary_wave(cel_engine *e, float H, float theta_deg, float x0, float y0);
 
+/* Terrain changes.  cel_engine_set_bottom replaces the bottom planes (BY, BX, BA as from
+ * cel_bottom_from_corners): fillTerrainTexture (engine.cpp:3313-3379) with keep_depth != 0
+ * (w becomes (w - BA_old) + BA_new everywhere, so water depths stay), fillTerrainTextureLite
+ * (engine.cpp:3381-3412) with keep_depth == 0 (the water stays).  Both clear the visual
+ * record; the line-solve coefficients and the depth-dependent boundary tables follow the
+ * new bottom; the time-stepping history is kept. */
+int cel_engine_set_bottom(cel_engine *e, const float *bottom3, int keep_depth);
+/* raiseLowerTerrain (engine.cpp:4685-4843): the Gaussian brush 2 exp(-z / (2 (1/3)^2)),
+ * z = |p|^2 / (2 radius)^2 <= 1, times displacement, sampled at the cell corners, is added to
+ * w and BA (the mean of the four corners), BX and BY (the means of the face's two corners)
+ * over the brush's index box.  World coordinates: x about the domain centre
+ * (index (nx-1)/W x + (nx+4)/2), y from the south (index (ny-1)/L y + 2.5), metres.
+ * Negative displacement lowers. */
+int cel_engine_raise_terrain(cel_engine *e, float x, float y, float radius, float displacement);
+/* shiftTerrainSlider (engine.cpp:4845-4924), as resetTimestep applies a Tide/Surge/SLR change
+ * (engine.cpp:1544-1553): cells with w - BA < 0.01 m become still water (BA below the still
+ * water level) or move with the terrain; BY, BX, BA move by shift_up; the visual record is
+ * cleared. */
+int cel_engine_shift_terrain(cel_engine *e, float shift_up);
+/* The left-mouse water brush (applyMouseShader, engine.cpp:4561-4683; left_mouse.hlsl:59-103)
+ * at world (x, y) with the given radius: the cells whose centres the reference's 12-triangle
+ * fan covers (D3D11 rasterisation: vertices snapped to 1/256 of a cell, top-left rule) take
+ * dh = disp_A + disp_B r, r the interpolated radius (0 at the centre, 1 on the rim):
+ * w' = max(B, w + dh), (hu, hv)' = (w' - B) (hu, hv) / max(0.5, w - B).  The reference's
+ * actions at strength s (= "left_mouse_strength" x dt): add water (s, 0), remove water
+ * (-s, 0), stir (-s, 1.5 s). */
+int cel_engine_edit_water(cel_engine *e, float x, float y, float radius, float disp_A, float disp_B);
+
 /* Runs nsteps and returns the elapsed device time in ms, measured with HIP events
  * on the engine's stream (inputs are already resident in HBM). */
 int cel_engine_timed_steps(cel_engine *e, int nsteps, float *ms);

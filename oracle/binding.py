@@ -63,6 +63,11 @@ def lib():
             getattr(L, f"orc_{p}_get_normals").argtypes = [ctypes.c_void_p, R]
             getattr(L, f"orc_{p}_add_solitary_wave").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4 + [ctypes.c_int]
             getattr(L, f"orc_{p}_refresh").argtypes = [ctypes.c_void_p]
+            getattr(L, f"orc_{p}_set_bottom").argtypes = [ctypes.c_void_p, _F, ctypes.c_int]
+            getattr(L, f"orc_{p}_raise_terrain").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4 + [ctypes.c_int]
+            getattr(L, f"orc_{p}_shift_terrain").argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]
+            getattr(L, f"orc_{p}_edit_water").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 5 + [ctypes.c_int]
+            getattr(L, f"orc_{p}_get_bottom").argtypes = [ctypes.c_void_p, R]
         L.orc_bottom_from_corners.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
         L.orc_f32_minmod.restype = ctypes.c_float
         L.orc_f32_minmod.argtypes = [ctypes.c_float] * 3
@@ -177,6 +182,24 @@ class Oracle:
     def refresh(self):
         """Pass1 of the current state (H, U, V and the normal), as the bootstrap Pass1."""
         getattr(lib(), f"orc_{self.p}_refresh")(self.h)
+
+    def set_bottom(self, bottom, keep_depth=True):
+        bottom = np.ascontiguousarray(bottom, np.float32)
+        getattr(lib(), f"orc_{self.p}_set_bottom")(self.h, fptr(bottom), int(keep_depth))
+
+    # refresh=True re-runs Pass1 on the edited state, as the engine's next step sees it; the
+    # reference's next step still uses the fluxes' inputs from before the edit (refresh=False)
+    def raise_terrain(self, x, y, radius, displacement, refresh=False):
+        getattr(lib(), f"orc_{self.p}_raise_terrain")(self.h, x, y, radius, displacement, int(refresh))
+
+    def shift_terrain(self, shift_up, refresh=False):
+        getattr(lib(), f"orc_{self.p}_shift_terrain")(self.h, shift_up, int(refresh))
+
+    def edit_water(self, x, y, radius, disp_A, disp_B=0.0, refresh=False):
+        getattr(lib(), f"orc_{self.p}_edit_water")(self.h, x, y, radius, disp_A, disp_B, int(refresh))
+
+    def bottom(self):
+        return self._planes("get_bottom")
 
     def add_solitary_wave(self, H, theta_deg, x0, y0, refresh=False):
         getattr(lib(), f"orc_{self.p}_add_solitary_wave")(self.h, H, theta_deg, x0, y0, int(refresh))

@@ -94,6 +94,15 @@ void orc_f32_get_normals(const orc_sim_f32 *s, float *out3);
 void orc_f32_add_solitary_wave(orc_sim_f32 *s, float H, float theta_deg, float x0, float y0, int refresh);
 /* Pass1 of the current state (as the bootstrap Pass1), refreshing H, U, V and the normal. */
 void orc_f32_refresh(orc_sim_f32 *s);
+/* Terrain and water edits: fillTerrainTexture(Lite), raiseLowerTerrain, shiftTerrainSlider
+ * and the left-mouse water brush (see oracle_impl.inc).  refresh != 0 re-runs Pass1 on the
+ * edited state; the reference does not, so its next step's fluxes still see the old state. */
+void orc_f32_set_bottom(orc_sim_f32 *s, const float *bottom3, int keep_depth);
+void orc_f32_raise_terrain(orc_sim_f32 *s, float x, float y, float radius, float displacement, int refresh);
+void orc_f32_shift_terrain(orc_sim_f32 *s, float shift_up, int refresh);
+void orc_f32_edit_water(orc_sim_f32 *s, float x, float y, float radius, float disp_A, float disp_B, int refresh);
+/* The bottom planes (BY, BX, BA) as the oracle holds them. */
+void orc_f32_get_bottom(const orc_sim_f32 *s, float *bottom3);
 
 orc_sim_f64 *orc_f64_create(const orc_config *cfg, const float *bottom, const float *state);
 void orc_f64_destroy(orc_sim_f64 *s);
@@ -111,6 +120,11 @@ void orc_f64_get_aux(const orc_sim_f64 *s, double *out3);
 void orc_f64_get_normals(const orc_sim_f64 *s, double *out3);
 void orc_f64_add_solitary_wave(orc_sim_f64 *s, float H, float theta_deg, float x0, float y0, int refresh);
 void orc_f64_refresh(orc_sim_f64 *s);
+void orc_f64_set_bottom(orc_sim_f64 *s, const float *bottom3, int keep_depth);
+void orc_f64_raise_terrain(orc_sim_f64 *s, float x, float y, float radius, float displacement, int refresh);
+void orc_f64_shift_terrain(orc_sim_f64 *s, float shift_up, int refresh);
+void orc_f64_edit_water(orc_sim_f64 *s, float x, float y, float radius, float disp_A, float disp_B, int refresh);
+void orc_f64_get_bottom(const orc_sim_f64 *s, double *bottom3);
 
 /* Terrain preprocessing, terrain_heightfield.cpp:244-298: corner heights
  * z[(ny+5)*(nx+5)] -> bottom planes BY, BX, BA. */

@@ -20,6 +20,7 @@ typedef float real;
 #define FABS fabsf
 #define ATAN atanf
 #define COSH coshf
+#define EXP expf
 #include "oracle_impl.inc"
 #undef FN
 #undef SIM
@@ -31,6 +32,9 @@ typedef float real;
 #undef FABS
 #undef ATAN
 #undef COSH
+#undef EXP
+#undef MAXI
+#undef MINI
 #undef NUM_TEX
 #undef MAXLEV
 #undef CURRENT_STATE
@@ -46,6 +50,7 @@ typedef float real;
 #define FABS fabs
 #define ATAN atan
 #define COSH cosh
+#define EXP exp
 #include "oracle_impl.inc"
 #undef real
 
