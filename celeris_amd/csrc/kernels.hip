@@ -879,13 +879,14 @@ __device__ __forceinline__ Val pre_value(const BoundaryParams &p, int x, int y)
 __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, int mirror_pass)
 {
     // enumerate the boundary cell set: left band, right band, then bottom/top bands between them
+    // (x fastest inside each band: the row-major w and hu accesses of a wave share rows)
     int x, y;
     {
         int r = tid;
         const int nl = p.band_l * p.Y, nr = p.band_r * p.Y;
         const int mid = p.X - p.band_l - p.band_r;
-        if (r < nl) { x = r / p.Y; y = r % p.Y; }
-        else if ((r -= nl) < nr) { x = p.X - p.band_r + r / p.Y; y = r % p.Y; }
+        if (r < nl) { y = r / p.band_l; x = r % p.band_l; }
+        else if ((r -= nl) < nr) { y = r / p.band_r; x = p.X - p.band_r + r % p.band_r; }
         else {
             r -= nr;
             const int nb = p.band_b * mid;
