@@ -16,7 +16,9 @@ a time.
 
 --workload ensemble prints the batched-ensemble line instead (64 scenarios with sides
 log-uniform in [128, 1024], compensated update, 50-step graph blocks) against the same
-scenarios run one at a time with the plain engine.
+scenarios run one at a time with the plain engine.  --workload precision runs the
+precision case (4096 x 1024 solitary wave on a beach, 100000 steps) in FP32, compensated
+FP32 and FP64 and reports each one's error against the FP64 run.
 """
 import argparse
 import json
@@ -136,6 +138,54 @@ def ensemble_line(args):
     }
 
 
+def precision_line(args):
+    """The precision case at its full size: 4096 x 1024 channel (dx 0.25 m, 1:50 beach to
+    +0.5 m, solitary wave H/h = 0.3, walls), run in FP32, compensated FP32 and FP64 on the GPU.
+    The FP64 engine is the reference (it equals the FP64 CPU oracle to round-off at sizes the
+    oracle can run); each variant reports its eta error over H at the end, its maximum
+    run-up on the centre row, its volume drift and its throughput."""
+    from celeris_amd import STATE_COMPENSATED, STATE_FP32, STATE_FP64, ShallowWaterEngine
+    from celeris_amd import scenarios as S
+    H, chunk = 0.3, 2000
+    sc = S.solitary_channel(nx=args.nx, ny=args.ny, dx=0.25, steps=args.steps)
+    row = sc.s.ny // 2 + 2
+    cells = sc.s.nx * sc.s.ny
+    xy = [(i, row) for i in range(sc.s.nx + 4)]
+    ba = sc.bottom[2][row].astype(np.float64)
+    out, finals = {}, {}
+    for name, prec in (("fp64", STATE_FP64), ("fp32", STATE_FP32), ("compensated", STATE_COMPENSATED)):
+        sc.s.state_precision = prec
+        e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        v0 = e.stats()["sum_h"]
+        runup, wall, done = -1e9, 0.0, 0
+        while done < args.steps:
+            k = min(chunk, args.steps - done)
+            t0 = time.perf_counter()
+            e.timestep(k)
+            e.sync()
+            wall += time.perf_counter() - t0
+            done += k
+            w = e.sample(xy)[:, 0].astype(np.float64)
+            wet = (w - ba) > 1e-3
+            if wet.any():
+                runup = max(runup, float(ba[wet].max()))
+            print(f"bench: precision {name} {done}/{args.steps}", file=sys.stderr, flush=True)
+        finals[name] = e.get_state_f64()
+        out[name] = {"gcell_steps_per_s": cells * args.steps / wall / 1e9, "runup_m": runup,
+                     "volume_drift": (e.stats()["sum_h"] - v0) / v0}
+        e.close()
+    ref = finals["fp64"][0][2:-2, 2:-2]
+    for name in ("fp32", "compensated", "fp64"):
+        d = finals[name][0][2:-2, 2:-2] - ref
+        out[name]["eta_linf_over_H"] = float(np.abs(d).max() / H)
+        out[name]["eta_l2_over_H"] = float(np.sqrt((d ** 2).mean()) / H)
+    return {"metric": "precision case (eta error vs GPU FP64)", "value": out["compensated"]["eta_linf_over_H"],
+            "unit": "L-inf eta error / H (compensated)", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+            "higher_is_better": False, "vs_baseline": None, "dtype": "f32 / f32+f32 / f64", "data": "synthetic",
+            "config": {"workload": f"solitary wave H/h 0.3 on a 1:50 beach, {sc.s.nx}x{sc.s.ny} cells, dx 0.25 m, "
+                                   f"dt {sc.s.timestep:.5f} s, walls", "variants": out}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +201,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
-    ap.add_argument("--workload", choices=["headline", "ensemble"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "ensemble", "precision"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
@@ -180,6 +230,13 @@ def main():
               "(multi-GPU sharding of the step is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "precision":
+        if "--steps" not in sys.argv:
+            args.steps = 100000
+        if "--nx" not in sys.argv:
+            args.nx, args.ny = 4096, 1024
+        print(json.dumps(precision_line(args)), flush=True)
+        return
     if args.workload == "ensemble":
         if args.graph_steps == 0 and "--graph-steps" not in sys.argv:
             args.graph_steps = 50
