@@ -10,9 +10,9 @@ resident in HBM before the timed region.
 Prints one JSON line: Gcell-steps/s (interior cells x steps / wall time), the dominant
 kernel's achieved HBM bandwidth against MI355X's 8 TB/s, the CPU oracle timed on a
 bounded sample of the same workload, and a bounded sample of the serving stream (Poisson
-arrivals of scenarios with sides 256-2048 and 500-20000 steps, continuous batching) with
-p50/p99 completion latency beside the plain engine serving the same stream one request at
-a time.
+arrivals of scenarios with sides 256-2048 and 500-20000 steps, continuous batching with
+all resident requests advancing, and with shortest-remaining-work-first) with p50/p99
+completion latency beside the plain engine serving the same stream one request at a time.
 
 --workload ensemble prints the batched-ensemble line instead (64 scenarios with sides
 log-uniform in [128, 1024], compensated update, 50-step graph blocks) against the same
@@ -88,9 +88,12 @@ def serving_leg(n, rates, seed, block):
     for rate in rates:
         reqs = [S.Request(r.rid, r.arrival / rate, r.scenario, r.steps) for r in base]
         _, batched, _ = serving.serve(reqs, block=block)
+        _, srpt, _ = serving.serve(reqs, block=block, policy="srpt")
         _, alone = serving.one_at_a_time(reqs, run_times=alone_times)
-        out["rates"][f"{rate:g}/s"] = {"batched": batched, "one_at_a_time": alone,
-                                       "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
+        out["rates"][f"{rate:g}/s"] = {
+            "batched": batched, "batched_srpt": srpt, "one_at_a_time": alone,
+            "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"],
+            "throughput_ratio_srpt": srpt["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
     return out
 
 

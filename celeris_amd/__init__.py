@@ -109,6 +109,7 @@ SIGNATURES = {
     "cel_batch_add": ([ctypes.c_void_p, ctypes.POINTER(CelSettings), ctypes.POINTER(CelBoundary), _F, _F,
                        ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_batch_remove": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "cel_batch_set_paused": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "cel_batch_step": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "cel_batch_set_graph_steps": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "cel_batch_sync": ([ctypes.c_void_p], ctypes.c_int),
@@ -443,6 +444,10 @@ class Batch:
     def remove(self, sid):
         _check(load_library().cel_batch_remove(self._h, sid))
         self._dims.pop(sid, None)
+
+    def set_paused(self, sid, paused=True):
+        """A paused scenario keeps its state and is skipped by step() until resumed."""
+        _check(load_library().cel_batch_set_paused(self._h, sid, int(paused)))
 
     def step(self, n=1):
         _check(load_library().cel_batch_step(self._h, n))

@@ -54,6 +54,17 @@ void Batch::remove(int id)
     if (it == scen_.end()) throw Error(CEL_EINVAL, "no such scenario");
     sync();
     scen_.erase(it);
+    paused_.erase(id);
+    comp_valid_ = false;
+}
+
+void Batch::setPaused(int id, bool paused)
+{
+    if (!scen_.count(id)) throw Error(CEL_EINVAL, "no such scenario");
+    if (paused == (paused_.count(id) != 0)) return;
+    sync();
+    if (paused) paused_.insert(id);
+    else paused_.erase(id);
     comp_valid_ = false;
 }
 
@@ -205,10 +216,11 @@ void Batch::issue(const char *blob, size_t stride, int m)
 
 void Batch::prepare(int m, bool advance, std::vector<StepLaunch> &plans, std::vector<ShallowWaterEngine *> &eng)
 {
-    const int ns = (int)scen_.size();
     std::vector<int> ids;
     eng.clear();
-    for (auto &kv : scen_) { ids.push_back(kv.first); eng.push_back(kv.second.get()); }
+    for (auto &kv : scen_)
+        if (!paused_.count(kv.first)) { ids.push_back(kv.first); eng.push_back(kv.second.get()); }
+    const int ns = (int)ids.size();
     plans.assign((size_t)m * ns, StepLaunch{});
     for (int k = 0; k < m; ++k)
         for (int i = 0; i < ns; ++i) {
@@ -273,7 +285,7 @@ void Batch::fillTable(const std::vector<StepLaunch> &plans, const std::vector<Sh
 
 void Batch::run(int m, bool use_graph)
 {
-    if (scen_.empty() || m == 0) return;
+    if (scen_.size() == paused_.size() || m == 0) return;    // nothing to advance
     std::vector<StepLaunch> plans;
     std::vector<ShallowWaterEngine *> eng;
     prepare(m, true, plans, eng);
@@ -317,7 +329,7 @@ int Batch::autotune(int repeats, bool tune_lines)
     sync();
     if (tune_lines)
         for (auto &kv : scen_) kv.second->autotune(repeats, nullptr);
-    if (scen_.empty()) return rows_;
+    if (scen_.size() == paused_.size()) return rows_;
     const int cand[3] = {4, 8, 16};
     float cand_ms[3] = {-1.f, -1.f, -1.f};
     for (int round = 0; round < 3; ++round)     // interleaved rounds, fastest round counts

@@ -12,6 +12,7 @@
 
 #include <map>
 #include <memory>
+#include <set>
 #include <vector>
 
 #include "engine.hpp"
@@ -34,6 +35,7 @@ public:
 
     int add(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3, const float *state3);
     void remove(int id);
+    void setPaused(int id, bool paused);     // paused scenarios are skipped by step()
     void step(int nsteps);
     void setGraphSteps(int k);
     int autotune(int repeats, bool tune_lines);
@@ -67,6 +69,7 @@ private:
 
     hipStream_t stream_ = nullptr;
     std::map<int, std::unique_ptr<ShallowWaterEngine>> scen_;
+    std::set<int> paused_;
     int next_id_ = 1;
     int graph_steps_ = 0;
     int rows_ = kDefaultBandRows;              // band rows of the batched explicit launch

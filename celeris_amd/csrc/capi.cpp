@@ -285,6 +285,7 @@ int cel_batch_add(cel_batch *b, const cel_settings *s, const cel_boundary bnd[4]
 }
 
 int cel_batch_remove(cel_batch *b, int id) { return guarded([&] { bat(b).remove(id); }); }
+int cel_batch_set_paused(cel_batch *b, int id, int paused) { return guarded([&] { bat(b).setPaused(id, paused != 0); }); }
 int cel_batch_step(cel_batch *b, int nsteps) { return guarded([&] { bat(b).step(nsteps); }); }
 int cel_batch_set_graph_steps(cel_batch *b, int steps) { return guarded([&] { bat(b).setGraphSteps(steps); }); }
 int cel_batch_sync(cel_batch *b) { return guarded([&] { bat(b).sync(); }); }

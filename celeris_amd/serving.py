@@ -7,6 +7,13 @@ HBM budget (first come, first served) and retires finished ones, so requests tha
 while others run join the running batch.  A request's latency is its completion time
 minus its arrival time.
 
+Two policies decide which resident requests advance in a block.  "fair" advances all of
+them (processor sharing).  "srpt" advances the requests with the least remaining work
+(cells x remaining steps) first, taking them in that order until their cells fill one
+large grid's worth of the GPU (gpu_cells); the rest stay resident but paused.  Small and
+nearly finished requests then complete without waiting for large ones, while the batch
+still holds enough cells to keep the GPU busy.
+
 The comparison point is the plain engine serving the same stream one request at a time
 (first come, first served): each request's run time alone is measured on the GPU and the
 queue is replayed from those times.
@@ -45,12 +52,14 @@ def summarize(records, makespan):
 
 
 def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=False, graph_steps=0,
-          clock=time.perf_counter, sleep=time.sleep):
+          policy="fair", gpu_cells=2048 * 2048, clock=time.perf_counter, sleep=time.sleep):
     """Runs the request stream in real time through one Batch, synchronising every `block`
     steps (or sooner, when a request completes).  graph_steps > 0 replays captured blocks
-    of that many steps (on MI355X direct launches measured faster).  Returns (records,
-    summary, states) where states maps request id to its final state (FP64 view of the
-    state, if keep_states)."""
+    of that many steps (on MI355X direct launches measured faster).  policy "fair" or
+    "srpt" (see the module notes).  Returns (records, summary, states) where states maps
+    request id to its final state (FP64 view of the state, if keep_states)."""
+    if policy not in ("fair", "srpt"):
+        raise ValueError("policy must be 'fair' or 'srpt'")
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     queue = deque()
     active = {}                    # batch id -> [request, remaining steps, bytes, record]
@@ -80,11 +89,21 @@ def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
-            k = min(block, min(v[1] for v in active.values()))
+            run = list(active)
+            if policy == "srpt":
+                run, cells = [], 0
+                for sid in sorted(active, key=lambda i: active[i][3].cells * active[i][1]):
+                    if run and cells + active[sid][3].cells > gpu_cells:
+                        break
+                    run.append(sid)
+                    cells += active[sid][3].cells
+                for sid in active:
+                    b.set_paused(sid, sid not in run)
+            k = min(block, min(active[sid][1] for sid in run))
             b.step(k)
             b.sync()
             done_at = clock() - t0
-            for sid in list(active):
+            for sid in run:
                 v = active[sid]
                 v[1] -= k
                 if v[1] == 0:

@@ -291,7 +291,10 @@ int cel_batch_destroy(cel_batch *b);
 int cel_batch_add(cel_batch *b, const cel_settings *s, const cel_boundary bnd[4], const float *bottom3,
                   const float *state3, int *id);
 int cel_batch_remove(cel_batch *b, int id);
-/* Advances every scenario by nsteps (asynchronous). */
+/* A paused scenario keeps its state and is skipped by cel_batch_step until resumed
+ * (paused = 0): a scheduler runs a chosen subset of the resident scenarios. */
+int cel_batch_set_paused(cel_batch *b, int id, int paused);
+/* Advances every scenario that is not paused by nsteps (asynchronous). */
 int cel_batch_step(cel_batch *b, int nsteps);
 /* Steps per captured block (0: direct launches). */
 int cel_batch_set_graph_steps(cel_batch *b, int steps);

@@ -78,3 +78,40 @@ def test_one_at_a_time_is_fifo():
     for a, b in zip(recs, recs[1:]):
         assert b.admitted == pytest.approx(max(b.arrival, a.finished))
     assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
+
+
+@pytest.mark.gpu
+def test_srpt_results_equal_alone_and_short_requests_go_first():
+    """The shortest-remaining-work policy pauses scenarios between blocks; every result is
+    still bit-identical to the request run alone, and with room for one scenario at a time
+    (gpu_cells 1) the requests complete in order of their work."""
+    reqs = [S.Request(r.rid, 0.0, r.scenario, r.steps) for r in small_stream(n=6, seed=6)]   # all at once
+    recs, summ, states = serving.serve(reqs, block=16, keep_states=True, policy="srpt", gpu_cells=1)
+    assert summ["requests"] == len(reqs)
+    for r in reqs:
+        assert np.array_equal(states[r.rid], alone_state(r)), f"request {r.rid} differs under srpt"
+    work = {r.rid: r.scenario.s.nx * r.scenario.s.ny * r.steps for r in reqs}
+    order = [rec.rid for rec in sorted(recs, key=lambda rec: rec.finished)]
+    assert [work[i] for i in order] == sorted(work[i] for i in order)
+
+
+@pytest.mark.gpu
+def test_paused_scenario_resumes_unchanged():
+    from celeris_amd import Batch
+    reqs = small_stream(n=2, seed=7)
+    b = Batch()
+    ids = [b.add(r.scenario.s, r.scenario.boundaries, r.scenario.bottom, r.scenario.state) for r in reqs]
+    b.step(10)
+    b.set_paused(ids[0])
+    b.step(25)                                   # only the second advances
+    b.set_paused(ids[0], False)
+    b.step(5)
+    t0, t1 = b.time(ids[0]), b.time(ids[1])
+    s0 = b.get_state_f64(ids[0])
+    b.close()
+    a = reqs[0].scenario
+    e = ShallowWaterEngine(a.s, a.boundaries, a.bottom, a.state)
+    e.timestep(15)
+    assert np.array_equal(s0, e.get_state_f64()) and t0 == pytest.approx(e.time, rel=0, abs=0)
+    assert t1 > t0
+    e.close()
