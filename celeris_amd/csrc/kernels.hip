@@ -75,12 +75,15 @@ __device__ __forceinline__ real from_right(real v) { return dpp_move<0x130>(v); 
 __device__ __forceinline__ real from_left(real v) { return dpp_move<0x138>(v); }    // wave_shr:1
 
 // MinMod, compute.hlsl:292-296: the minimum of three positive slopes, the maximum of three
-// negative ones, else zero -- selected through min3/max3.
+// negative ones, else zero.  That is the median of (0, min, max): min when all three are
+// positive, max when all are negative, 0 otherwise -- one v_med3 after min3/max3.
+__device__ __forceinline__ float MED3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+__device__ __forceinline__ double MED3(double a, double b, double c) { return fmax(fmin(a, b), fmin(fmax(a, b), c)); }
 __device__ __forceinline__ real minmod3(real a, real b, real c)
 {
     const real mn = FMIN(FMIN(a, b), c);
     const real mx = FMAX(FMAX(a, b), c);
-    return mn > 0.f ? mn : (mx < 0.f ? mx : 0.f);
+    return MED3(real(0), mn, mx);
 }
 
 // Reconstruct, compute.hlsl:298-315: half-slope of the generalized minmod limiter
