@@ -218,7 +218,6 @@ __device__ __forceinline__ void line_abc(real K1, real Q, bool first, bool last,
 // Per-cell inputs of Pass3 that are read once: Adams-Bashforth history and static fields.
 struct CellIn {
     real o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
-    real Kx, Qx, Ky, Qy, ddx, ddy;
     real wlo;      // low word of w (compensated state)
 };
 
@@ -234,9 +233,6 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
     } else {
         c.o0 = c.o1 = c.o2 = c.oo0 = c.oo1 = c.oo2 = c.Fo = c.Go = c.Foo = c.Goo = 0.f;
     }
-    c.Kx = ld32(p.Kx, i - 2u); c.Qx = ld32(p.Qx, i - 2u);
-    c.Ky = ld32(p.Ky, i - 2u); c.Qy = ld32(p.Qy, i - 2u);
-    c.ddx = ld32(p.DDx, i - 2u); c.ddy = ld32(p.DDy, i - 2u);
     c.wlo = COMP ? ld32(p.w_lo, i) : 0.f;
     return c;
 }
@@ -297,7 +293,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     const real sl = p.sea_level;
 
     // register windows over rows; index 0 = y-2 ... 4 = y+2
-    real w_[5], hu_[4], hv_[4], BY_[4];
+    real w_[5], BA_[5], hu_[4], hv_[4], BY_[4];
     // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
     // 32-bit element offsets: planes are far below 4 G elements, and a uniform base with a
     // 32-bit offset maps to the scalar-base addressing mode (no 64-bit address arithmetic)
@@ -307,6 +303,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     for (int j = 0; j < 4; ++j) {
         const int row = ys - 2 + j;
         w_[j] = ldrow(p.w, row);
+        BA_[j] = ldrow(p.BA, row);
     }
     for (int j = 0; j < 3; ++j) {
         const int row = ys - 1 + j;
@@ -334,7 +331,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     // software pipelining: the loads a row needs are issued one row ahead
     const int ylast = p.ny + 3;
     real nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
-    real nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys);
+    real nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys + 2);
     CellIn nx_c = load_cell<SCHEME, COMP>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
 
     for (int y = ys; y < ye; ++y) {
@@ -344,7 +341,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         hv_[3] = hvblk[y + 2 - (ys - 2)][lane];
         BY_[3] = nx_BY;
         const real BXc = nx_BX;
-        const real BAc = nx_BA;
+        BA_[4] = nx_BA;
+        const real BAc = BA_[2];
         const CellIn cin = nx_c;
         {
             const int yn = min(y + 3, ylast), yc = min(y + 1, ye - 1);
@@ -352,7 +350,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             nx_hu = ldrow(p.hu, yn);
             nx_BY = ldrow(p.BY, yn);
             nx_BX = ldrow(p.BX, yc);
-            nx_BA = ldrow(p.BA, yc);
+            nx_BA = ldrow(p.BA, yn);
             nx_c = load_cell<SCHEME, COMP>(p, (unsigned)yc * (unsigned)P + (unsigned)xc);
         }
 
@@ -385,6 +383,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         const real wdl = from_left(w_[1]), wdr = from_right(w_[1]);
         const real BYs = BY_[0];
         const real BYc = BY_[1];
+        // the bed at x-2 .. x+2 for the line coefficients and the depth gradients
+        const real bal = from_left(BAc), bar = from_right(BAc);
+        const real ball = from_left(bal), barr = from_right(bar);
 
         if (out_lane) {
             const unsigned i = (unsigned)y * (unsigned)P + (unsigned)x;
@@ -407,8 +408,21 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             const real one_over_d2y = ody * ody;
             const real one_over_d3y = one_over_d2y * ody;
 
-            const real dd_by_dx = cin.ddx;
-            const real dd_by_dy = cin.ddy;
+            // K1, K2/72 of the x- and y-line through this cell and the fourth-order depth
+            // gradients, from the bed (the same operations as static_fields): recomputed
+            // here instead of read, which saves 24 bytes per cell of HBM traffic
+            const real cs = p.coef_sl;
+            real Kx, Qx, Ky, Qy;
+            line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal), fmaxr(0.f, cs - BAc),
+                    fmaxr(0.f, cs - bar), fmaxr(0.f, cs - barr), Kx, Qx);
+            line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - BA_[0]), fmaxr(0.f, cs - BA_[1]), fmaxr(0.f, cs - BA_[2]),
+                    fmaxr(0.f, cs - BA_[3]), fmaxr(0.f, cs - BA_[4]), Ky, Qy);
+            const real d_l = fmaxr(0.f, sl - bal), d_r = fmaxr(0.f, sl - bar);
+            const real d_ll = fmaxr(0.f, sl - ball), d_rr = fmaxr(0.f, sl - barr);
+            const real d_d = fmaxr(0.f, sl - BA_[1]), d_u = fmaxr(0.f, sl - BA_[3]);
+            const real d_dd = fmaxr(0.f, sl - BA_[0]), d_uu = fmaxr(0.f, sl - BA_[4]);
+            const real dd_by_dx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * odx / 12.f;
+            const real dd_by_dy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * ody / 12.f;
             const real eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
             const real eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
             const real eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
@@ -450,8 +464,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
 
             // level-0 coefficients of the x-line (row y) and y-line (column x) through this cell
             real Ax, Bx, Cx, Ay, By, Cy;
-            line_abc(cin.Kx, cin.Qx, x == 2, x == p.nx + 1, Ax, Bx, Cx);
-            line_abc(cin.Ky, cin.Qy, y == 2, y == p.ny + 1, Ay, By, Cy);
+            line_abc(Kx, Qx, x == 2, x == p.nx + 1, Ax, Bx, Cx);
+            line_abc(Ky, Qy, y == 2, y == p.ny + 1, Ay, By, Cy);
 
             const real dt = p.dt;
             real wOut, huOut, hvOut;
@@ -531,6 +545,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         }
         // ---- shift windows
         w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];
+        BA_[0] = BA_[1]; BA_[1] = BA_[2]; BA_[2] = BA_[3]; BA_[3] = BA_[4];
         hu_[0] = hu_[1]; hu_[1] = hu_[2]; hu_[2] = hu_[3];
         hv_[0] = hv_[1]; hv_[1] = hv_[2]; hv_[2] = hv_[3];
         BY_[0] = BY_[1]; BY_[1] = BY_[2]; BY_[2] = BY_[3];
@@ -1350,8 +1365,8 @@ __global__ void transpose_plane(const real *in, int pin, real *out, int pout, in
     }
 }
 
-// Static fields of every interior cell from the bottom (engine.cpp:2656-2680 for K1, K2/72;
-// compute.hlsl:696-697 for the depth gradients).
+// Line-coefficient fields of every interior cell from the bottom (engine.cpp:2656-2680: K1,
+// K2/72), row-major for the x-lines and transposed for the y-lines.
 __global__ void static_fields(const StaticParams p)
 {
     const int x = 2 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1359,20 +1374,14 @@ __global__ void static_fields(const StaticParams p)
     if (x >= p.nx + 2 || y >= p.ny + 2) return;
     const real *BA = p.BA;
     auto ba = [&](int xx, int yy) { return BA[(size_t)yy * p.P + xx]; };
-    const real cs = p.coef_sl, sl = p.sea_level;
+    const real cs = p.coef_sl;
     real Kx, Qx, Ky, Qy;
     line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ba(x - 2, y)), fmaxr(0.f, cs - ba(x - 1, y)), fmaxr(0.f, cs - ba(x, y)),
             fmaxr(0.f, cs - ba(x + 1, y)), fmaxr(0.f, cs - ba(x + 2, y)), Kx, Qx);
     line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - ba(x, y - 2)), fmaxr(0.f, cs - ba(x, y - 1)), fmaxr(0.f, cs - ba(x, y)),
             fmaxr(0.f, cs - ba(x, y + 1)), fmaxr(0.f, cs - ba(x, y + 2)), Ky, Qy);
-    const real d_l = fmaxr(0.f, sl - ba(x - 1, y)), d_r = fmaxr(0.f, sl - ba(x + 1, y));
-    const real d_ll = fmaxr(0.f, sl - ba(x - 2, y)), d_rr = fmaxr(0.f, sl - ba(x + 2, y));
-    const real d_d = fmaxr(0.f, sl - ba(x, y - 1)), d_u = fmaxr(0.f, sl - ba(x, y + 1));
-    const real d_dd = fmaxr(0.f, sl - ba(x, y - 2)), d_uu = fmaxr(0.f, sl - ba(x, y + 2));
-    const real ddx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * p.one_over_dx / 12.f;
-    const real ddy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * p.one_over_dy / 12.f;
     const size_t r = (size_t)y * p.P + (x - 2), c = (size_t)x * p.PT + (y - 2);
-    p.Kx[r] = Kx; p.Qx[r] = Qx; p.Ky[r] = Ky; p.Qy[r] = Qy; p.DDx[r] = ddx; p.DDy[r] = ddy;
+    p.Kx[r] = Kx; p.Qx[r] = Qx;
     p.KyT[c] = Ky; p.QyT[c] = Qy;
 }
 
@@ -1396,8 +1405,8 @@ __global__ void export_coefficients(const StaticParams p, real *ax, real *ay)
     A = 0.f; B = 1.f; C = 0.f;
     if (y >= 2 && y <= Y - 3) {
         const int xx = x < 2 ? 2 : (x > p.nx + 1 ? p.nx + 1 : x);
-        const size_t r = (size_t)y * p.P + (xx - 2);
-        line_abc(p.Ky[r], p.Qy[r], y == 2, y == Y - 3, A, B, C);
+        const size_t c = (size_t)xx * p.PT + (y - 2);
+        line_abc(p.KyT[c], p.QyT[c], y == 2, y == Y - 3, A, B, C);
     }
     ay[i] = A; ay[N + i] = B; ay[2 * N + i] = C;
 }

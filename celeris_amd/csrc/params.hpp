@@ -67,9 +67,9 @@ struct ExplicitParams {
     // variable-step AB3 coefficients (AdamsBashforthCoef) and derivative weights
     real ab[3];
     real der[9];
-    // static per-cell fields, [y * P + (x - 2)]: K1 and K2/72 of the x- and y-line
-    // coefficients, and the fourth-order depth gradients dd/dx, dd/dy
-    const real *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;
+    // line-coefficient constants (BCOEF + 1/3, dx^2, dy^2, the coefficient sea level): the
+    // kernel forms K1, K2/72 and the depth gradients from the bed as static_fields does
+    real kcoef, dx2, dy2, coef_sl;
 };
 
 // Static per-cell fields derived from the bottom once per mesh (bit-identical to computing
@@ -77,10 +77,9 @@ struct ExplicitParams {
 struct StaticParams {
     int nx, ny, P, PT;
     const real *BA;
-    real *Kx, *Qx, *Ky, *Qy, *DDx, *DDy;   // row-major, [y * P + (x - 2)]
-    real *KyT, *QyT;                       // transposed, [x * PT + (y - 2)]
+    real *Kx, *Qx;                         // x-lines, row-major, [y * P + (x - 2)]
+    real *KyT, *QyT;                       // y-lines, transposed, [x * PT + (y - 2)]
     real kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
-    real sea_level, one_over_dx, one_over_dy;
 };
 
 struct LineParams {
