@@ -1020,7 +1020,8 @@ void ShallowWaterEngine::setTuning(const cel_tuning &t)
             if (kShapeC[i] == c && kShapeW[i] == w) return i;
         return -1;
     };
-    if (t.band_rows != 4 && t.band_rows != 8 && t.band_rows != 16) throw Error(CEL_EINVAL, "band_rows must be 4, 8 or 16");
+    if (t.band_rows != 4 && t.band_rows != 8 && t.band_rows != 12 && t.band_rows != 16)
+        throw Error(CEL_EINVAL, "band_rows must be 4, 8, 12 or 16");
     const int sx = shape_of(t.x_chunk, t.x_waves), sy = shape_of(t.y_chunk, t.y_waves);
     if (sx < 0 || sy < 0) throw Error(CEL_EINVAL, "unsupported line-solve block shape");
     if (shape_capacity(sx) < nx_ || shape_capacity(sy) < ny_)
@@ -1055,10 +1056,11 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
     // candidates are timed in three interleaved rounds and judged by their fastest round,
     // so clock ramps and other transients do not pick the winner
     constexpr int kRounds = 3;
-    const int rows_cand[3] = {4, 8, 16};
-    float rows_us[3] = {-1.f, -1.f, -1.f};
+    constexpr int kRowsCand = 4;
+    const int rows_cand[kRowsCand] = {4, 8, 12, 16};
+    float rows_us[kRowsCand] = {-1.f, -1.f, -1.f, -1.f};
     for (int round = 0; round < kRounds; ++round)
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < kRowsCand; ++c) {
             rows_ = rows_cand[c];
             ExplicitParams p;
             fillExplicit(p);
@@ -1066,7 +1068,7 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
             if (rows_us[c] < 0.f || us < rows_us[c]) rows_us[c] = us;
         }
     int bc = 0;
-    for (int c = 1; c < 3; ++c)
+    for (int c = 1; c < kRowsCand; ++c)
         if (rows_us[c] < rows_us[bc]) bc = c;
     rows_ = rows_cand[bc];
     tuned_us_[0] = rows_us[bc];

@@ -1425,10 +1425,11 @@ hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams
 #define CEL_EXPLICIT_R(S, CP)                                  \
     do {                                                       \
         if (rows == 4) CEL_EXPLICIT(S, CP, 4);                 \
+        else if (rows == 12) CEL_EXPLICIT(S, CP, 12);          \
         else if (rows == 16) CEL_EXPLICIT(S, CP, 16);          \
         else CEL_EXPLICIT(S, CP, 8);                           \
     } while (0)
-    if (rows != 4 && rows != 8 && rows != 16) return hipErrorInvalidValue;
+    if (rows != 4 && rows != 8 && rows != 12 && rows != 16) return hipErrorInvalidValue;
     if (comp) {
         switch (scheme) {
         case 0: CEL_EXPLICIT_R(0, true); break;

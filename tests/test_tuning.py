@@ -31,14 +31,14 @@ def run(sc, steps, tuning=None, autotune=False):
     e.timestep(steps // 2)
     if autotune:
         t = e.autotune(repeats=3)
-        assert t["band_rows"] in (4, 8, 16) and t["explicit_us"] > 0
+        assert t["band_rows"] in (4, 8, 12, 16) and t["explicit_us"] > 0
     e.timestep(steps - steps // 2)
     st = e.get_state_f64()
     e.close()
     return st
 
 
-@pytest.mark.parametrize("tuning", [(4, 1, 2, 2, 2), (16, 2, 1, 1, 2), (8, 1, 2, 4, 1), (16, 4, 4, 8, 2)])
+@pytest.mark.parametrize("tuning", [(4, 1, 2, 2, 2), (16, 2, 1, 1, 2), (8, 1, 2, 4, 1), (16, 4, 4, 8, 2), (12, 2, 1, 2, 1)])
 def test_fixed_tunings_bit_identical(tuning):
     sc = case(100, 70, 41)
     assert np.array_equal(run(sc, 60), run(sc, 60, tuning=tuning))
