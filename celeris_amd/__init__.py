@@ -58,7 +58,7 @@ class CelAdaptive(ctypes.Structure):
 class CelTuning(ctypes.Structure):
     _fields_ = [("band_rows", ctypes.c_int), ("x_chunk", ctypes.c_int), ("x_waves", ctypes.c_int),
                 ("y_chunk", ctypes.c_int), ("y_waves", ctypes.c_int), ("explicit_us", ctypes.c_float),
-                ("x_lines_us", ctypes.c_float), ("y_lines_us", ctypes.c_float)]
+                ("x_lines_us", ctypes.c_float), ("y_lines_us", ctypes.c_float), ("fast_arith", ctypes.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -367,8 +367,10 @@ class ShallowWaterEngine:
         _check(load_library().cel_engine_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, band_rows, x_chunk, x_waves, y_chunk, y_waves):
-        t = CelTuning(band_rows, x_chunk, x_waves, y_chunk, y_waves, 0.0, 0.0, 0.0)
+    def set_tuning(self, band_rows, x_chunk, x_waves, y_chunk, y_waves, fast_arith=True):
+        """Fixes the work decomposition; fast_arith=False forbids the explicit kernel's
+        shortened exact division and square root (results are the same either way)."""
+        t = CelTuning(band_rows, x_chunk, x_waves, y_chunk, y_waves, 0.0, 0.0, 0.0, 1 if fast_arith else 0)
         _check(load_library().cel_engine_set_tuning(self._h, ctypes.byref(t)))
 
     def set_graph_steps(self, k):

@@ -141,9 +141,10 @@ void ShallowWaterEngine::allocate()
     const int tiles = ((4 * (nx_ / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (ny_ / 4) + kStatsTileY - 1) / kStatsTileY);
     stats_blocks_ = tiles;
     const size_t sbytes = ((size_t)tiles + 1) * kStatsValues * sizeof(double);
-    check(hipMalloc(&mem, sbytes), "hipMalloc(stats)");
+    check(hipMalloc(&mem, sbytes + 256), "hipMalloc(stats)");
     d_stats_ = static_cast<double *>(mem);
-    table_bytes_ += sbytes;
+    d_flag_ = reinterpret_cast<int *>(static_cast<char *>(mem) + sbytes);   // after the partial sums
+    table_bytes_ += sbytes + 256;
     check(hipHostMalloc(&mem, kStatsValues * sizeof(double), hipHostMallocDefault), "hipHostMalloc(stats)");
     h_stats_ = static_cast<double *>(mem);
 }
@@ -156,7 +157,7 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
     const size_t plane = P * Y * sizeof(real), tplane = PT * X * sizeof(real);
     size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 2 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 2 + (comp ? 2 : 0));
     const int tiles = ((4 * (s.nx / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (s.ny / 4) + kStatsTileY - 1) / kStatsTileY);
-    b += ((size_t)tiles + 1) * kStatsValues * sizeof(double);
+    b += ((size_t)tiles + 1) * kStatsValues * sizeof(double) + 256;
     for (int side = 0; side < 4; ++side) {
         const cel_boundary &c = bnd[side];
         const size_t ncell = (size_t)c.width * ((side == CEL_WEST || side == CEL_EAST) ? Y : X);
@@ -194,8 +195,25 @@ void ShallowWaterEngine::uploadHostBottom()
     hostBA_.assign(b.begin() + 2 * N, b.begin() + 3 * N);
     StaticParams sp;
     fillStatic(sp);
+    check(hipMemsetAsync(d_flag_, 0, sizeof(int), stream_), "clear flag");
     check(launch_static_fields(sp, stream_), "static_fields");
+    int unsafe = 1;
+    check(hipMemcpyAsync(&unsafe, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read flag");
     check(hipStreamSynchronize(stream_), "upload sync");
+    fast_mesh_ = unsafe == 0 && fastSettings();
+}
+
+// The explicit kernel's FAST variant needs: FP32 state, epsilon >= 2^-96 (the CalcUV root
+// argument), dx^2 and dy^2 in [2^-20, 2^20], and every numerator of its divisions by dx^2,
+// dy^2, 72 and 12 in [2^-80, 2^80] or zero (checked per cell by static_fields).  Its
+// results are then bit-identical to the plain kernel's.
+bool ShallowWaterEngine::fastSettings() const
+{
+    if (sizeof(real) != 4) return false;
+    ExplicitParams p;
+    fillExplicit(p);
+    auto in = [](real v) { return v >= real(0x1p-20f) && v <= real(0x1p20f); };
+    return p.epsilon >= real(0x1p-96f) && in(p.dx2) && in(p.dy2);
 }
 
 // createSimTextures ring/scheme init, engine.cpp:3254-3269, and resetTimestep's first
@@ -585,6 +603,8 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     p.dx2 = dx_ * dx_;
     p.dy2 = dy_ * dy_;
     p.coef_sl = s_.is_boussinesq ? (real)s_.still_water_elevation : real(-9999);
+    p.rdx2 = real(1) / p.dx2;
+    p.rdy2 = real(1) / p.dy2;
 }
 
 // createTridiagonalCoefTextures' level-0 inputs (engine.cpp:2619-2625) and the depth
@@ -599,6 +619,12 @@ void ShallowWaterEngine::fillStatic(StaticParams &p) const
     p.dx2 = dx_ * dx_;
     p.dy2 = dy_ * dy_;
     p.coef_sl = s_.is_boussinesq ? (real)s_.still_water_elevation : real(-9999);
+    p.rdx2 = real(1) / p.dx2;
+    p.rdy2 = real(1) / p.dy2;
+    p.sl = s_.still_water_elevation;
+    p.odx = real(1) / dx_;
+    p.ody = real(1) / dy_;
+    p.unsafe = d_flag_;
 }
 
 void ShallowWaterEngine::fillLines(LineParams &p) const
@@ -739,7 +765,7 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const real *dclock, bool
     }
     // the visual outputs read only the step's starting state, which no kernel of the step writes
     if (L.aux) check(launch_aux(L.ap, stream_), "aux_update");
-    check(launch_explicit(L.scheme, comp_, L.rows, L.ep, stream_), "explicit_step");
+    check(launch_explicit(L.scheme, comp_, L.rows, fastArith(), L.ep, stream_), "explicit_step");
     if (profile) check(hipEventRecord(pe[1], stream_), "event");
     for (int i = 0; i < L.ncopy; ++i)
         check(hipMemcpyAsync(L.copy[i].dst, L.copy[i].src, L.copy[i].bytes, hipMemcpyDeviceToDevice, stream_), "copy");
@@ -1011,6 +1037,7 @@ void ShallowWaterEngine::getTuning(cel_tuning *out) const
     out->x_chunk = kShapeC[shape_x_]; out->x_waves = kShapeW[shape_x_];
     out->y_chunk = kShapeC[shape_y_]; out->y_waves = kShapeW[shape_y_];
     out->explicit_us = tuned_us_[0]; out->x_lines_us = tuned_us_[1]; out->y_lines_us = tuned_us_[2];
+    out->fast_arith = fastArith() ? 1 : 0;
 }
 
 void ShallowWaterEngine::setTuning(const cel_tuning &t)
@@ -1031,6 +1058,7 @@ void ShallowWaterEngine::setTuning(const cel_tuning &t)
     rows_ = t.band_rows;
     shape_x_ = sx;
     shape_y_ = sy;
+    fast_allowed_ = t.fast_arith != 0;
     tuned_us_[0] = tuned_us_[1] = tuned_us_[2] = 0.f;
 }
 
@@ -1064,7 +1092,7 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
             rows_ = rows_cand[c];
             ExplicitParams p;
             fillExplicit(p);
-            const float us = time_it([&] { check(launch_explicit(scheme, comp_, rows_, p, stream_), "explicit_step"); });
+            const float us = time_it([&] { check(launch_explicit(scheme, comp_, rows_, fastArith(), p, stream_), "explicit_step"); });
             if (rows_us[c] < 0.f || us < rows_us[c]) rows_us[c] = us;
         }
     int bc = 0;

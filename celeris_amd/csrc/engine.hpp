@@ -30,7 +30,7 @@ namespace CEL_NS {
 using cel::Error;
 #endif
 
-hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st);
+hipError_t launch_explicit(int scheme, bool comp, int rows, bool fast, const ExplicitParams &p, hipStream_t st);
 hipError_t launch_solve(bool comp, int shape_x, int shape_y, const LineParams &p, const IrregularAll &irr,
                         hipStream_t st);
 int default_shape(int n);
@@ -206,6 +206,12 @@ private:
     // explicit band rows and line-solve shapes (indices into kShapeC / kShapeW)
     int rows_ = kDefaultBandRows, shape_x_ = 0, shape_y_ = 0;
     float tuned_us_[3] = {};
+    // the explicit kernel's FAST variant (shortened exact division and square root): the
+    // mesh and settings admit it (static_fields' check, fastSettings) and the caller allows it
+    int *d_flag_ = nullptr;
+    bool fast_mesh_ = false, fast_allowed_ = true;
+    bool fastSettings() const;
+    bool fastArith() const { return fast_mesh_ && fast_allowed_; }
 
     // scheme state (engine.cpp:3254-3269, 1476-1519)
     int cur_ = 0;

@@ -40,6 +40,46 @@ __device__ __forceinline__ double FABS(double x) { return fabs(x); }
 __device__ __forceinline__ float POW(float x, float y) { return powf(x, y); }
 __device__ __forceinline__ double POW(double x, double y) { return pow(x, y); }
 
+// Correctly rounded quotient and square root in fewer instructions, for the FAST variants
+// of the explicit kernel (FP32; the FP64 build keeps the plain operations).  On the
+// arguments for which the engine selects them they return the bits of a / b and sqrtf
+// (up to the sign of a zero result, which never changes a nonzero value downstream):
+//   div_k(a, b, rb): b fixed for the launch, rb = RN(1/b).  q = a rb lies within one ulp of
+//     a / b, the remainder a - b q is exact in one FMA, and q + (a - b q) rb rounds to the
+//     correctly rounded quotient (Markstein's theorem) for a == 0 or 2^-80 <= |a| <= 2^80
+//     and 2^-20 <= b <= 2^20 (no intermediate leaves the normal range).  The library
+//     division spends about ten instructions on a general b.
+//   sqrt_k(x): x == 0 or 2^-96 <= x < inf.  The hardware root (within one ulp) is corrected
+//     by the sign of the residual on either neighbour, as the compiler's own expansion of
+//     sqrtf does for arguments that need no rescaling; the rescaling and the special-value
+//     tests it adds for the others are dropped.
+template <bool FAST>
+__device__ __forceinline__ real div_k(real a, real b, real rb)
+{
+#ifndef CEL_FP64
+    if (FAST) {
+        const float q = a * rb;
+        return __builtin_fmaf(__builtin_fmaf(-b, q, a), rb, q);
+    }
+#endif
+    (void)rb;
+    return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ real sqrt_k(real x)
+{
+#ifndef CEL_FP64
+    if (FAST) {
+        const float s = __builtin_amdgcn_sqrtf(x);
+        const float sm = __int_as_float(__float_as_int(s) - 1);
+        const float sp = __int_as_float(__float_as_int(s) + 1);
+        const float r = __builtin_fmaf(-sm, s, x) <= 0.f ? sm : s;
+        return __builtin_fmaf(-sp, s, x) > 0.f ? sp : r;
+    }
+#endif
+    return SQRT(x);
+}
+
 // 16 bytes of consecutive elements (4 floats or 2 doubles) and a pair of elements, for
 // vector loads and stores
 constexpr int kVec = 16 / (int)sizeof(real);
@@ -108,12 +148,13 @@ __device__ __forceinline__ void correct_w(real B_west, real B_east, real w_bar, 
     w_west = nw;
 }
 
-// desingularised 1/h of CalcUV, compute.hlsl:330-340
+// desingularised 1/h of CalcUV, compute.hlsl:330-340 (the root's argument is >= eps)
+template <bool FAST = false>
 __device__ __forceinline__ real divide_by_h(real h, real eps)
 {
     const real h2 = h * h;
     const real h4 = h2 * h2;
-    return SQRT(real(2)) * h / SQRT(h4 + fmaxr(h4, eps));
+    return SQRT(real(2)) * h / sqrt_k<FAST>(h4 + fmaxr(h4, eps));
 }
 
 // NumericalFlux, compute.hlsl:363-370
@@ -147,6 +188,7 @@ struct Face { real h, u, v; };
 
 // Face values of one cell along one axis (Pass1, compute.hlsl:408-433): returns the
 // "low" face (W or S) and the "high" face (E or N).
+template <bool FAST>
 __device__ __forceinline__ void faces_1d(real two_theta, real eps,
                                          real w_lo, real w_c, real w_hi,
                                          real hu_lo, real hu_c, real hu_hi,
@@ -162,10 +204,10 @@ __device__ __forceinline__ void faces_1d(real two_theta, real eps,
     correct_w(B_lo, B_hi, w_c, w_lo_f, w_hi_f);
     hi.h = w_hi_f - B_hi;
     lo.h = w_lo_f - B_lo;
-    const real qh = divide_by_h(hi.h, eps);
+    const real qh = divide_by_h<FAST>(hi.h, eps);
     hi.u = qh * hu_hi_f;
     hi.v = qh * hv_hi_f;
-    const real ql = divide_by_h(lo.h, eps);
+    const real ql = divide_by_h<FAST>(lo.h, eps);
     lo.u = ql * hu_lo_f;
     lo.v = ql * hv_lo_f;
 }
@@ -173,6 +215,7 @@ __device__ __forceinline__ void faces_1d(real two_theta, real eps,
 // Pass2 x-direction flux between cell E face (here) and the W face of the east cell.
 // normal velocity u, tangential v; for y-direction call with (v, u) swapped and the
 // momentum components mapped as in compute.hlsl:517-533.
+template <bool FAST>
 __device__ __forceinline__ void flux_x(real g, real half_g, const Face &E, const Face &Wn, real &f0, real &f1, real &f2)
 {
     const real cE = SQRT(g * E.h);
@@ -185,6 +228,7 @@ __device__ __forceinline__ void flux_x(real g, real half_g, const Face &E, const
     f2 = num_flux(ap, am, Wn.h * Wn.u * Wn.v, E.h * E.u * E.v, Wn.h * Wn.v - E.h * E.v);
 }
 
+template <bool FAST>
 __device__ __forceinline__ void flux_y(real g, real half_g, const Face &N, const Face &Sn, real &f0, real &f1, real &f2)
 {
     const real cN = SQRT(g * N.h);
@@ -198,13 +242,14 @@ __device__ __forceinline__ void flux_y(real g, real half_g, const Face &N, const
 }
 
 // Level-0 tridiagonal coefficients (engine.cpp:2664-2680) of a line cell: K1 and K2/72
-// from the depths at offsets -2..+2 along the line, then A, B, C.
-__device__ __forceinline__ void line_kq(real kcoef, real d2, real dm2, real dm1, real d0, real dp1, real dp2,
-                                        real &K1, real &Q)
+// from the depths at offsets -2..+2 along the line, then A, B, C.  rd2 = RN(1/d2).
+template <bool FAST = false>
+__device__ __forceinline__ void line_kq(real kcoef, real d2, real rd2, real dm2, real dm1, real d0, real dp1,
+                                        real dp2, real &K1, real &Q)
 {
-    K1 = kcoef * (d0 * d0 / d2);
-    const real K2 = (d0 / d2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
-    Q = K2 / 72.f;
+    K1 = kcoef * div_k<FAST>(d0 * d0, d2, rd2);
+    const real K2 = div_k<FAST>(d0, d2, rd2) * (-dp2 + 8.f * dp1 - 8.f * dm1 + dm2);
+    Q = div_k<FAST>(K2, real(72), real(1) / real(72));
 }
 __device__ __forceinline__ void line_abc(real K1, real Q, bool first, bool last, real &A, real &B, real &C)
 {
@@ -248,7 +293,7 @@ struct ExplicitSmem {
 
 // One wave computes interior columns [x0, x0 + 60) of rows [ys, ys + R) of one scenario.
 // SCHEME: 0 Euler, 1 AB3 predictor, 2 AM4 corrector, 3 variable-step AB3
-template <int SCHEME, bool COMP, int R, bool CS>
+template <int SCHEME, bool COMP, int R, bool CS, bool FAST = false>
 __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile, ExplicitSmem<CS, R> &sm)
 {
     real (*hvblk)[64] = sm.hv;
@@ -319,14 +364,14 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     Face Nprev, Scur, Ncur;
     {
         Face lo, hi;
-        faces_1d(tt, eps, w_[0], w_[1], w_[2], hu_m2, hu_[0], hu_[1], hv_m2, hv_[0], hv_[1], BY_m2, BY_[0], lo, hi);
+        faces_1d<FAST>(tt, eps, w_[0], w_[1], w_[2], hu_m2, hu_[0], hu_[1], hv_m2, hv_[0], hv_[1], BY_m2, BY_[0], lo, hi);
         Nprev = hi;
-        faces_1d(tt, eps, w_[1], w_[2], w_[3], hu_[0], hu_[1], hu_[2], hv_[0], hv_[1], hv_[2], BY_[0], BY_[1], lo, hi);
+        faces_1d<FAST>(tt, eps, w_[1], w_[2], w_[3], hu_[0], hu_[1], hu_[2], hv_[0], hv_[1], hv_[2], BY_[0], BY_[1], lo, hi);
         Scur = lo;
         Ncur = hi;
     }
     real yf_s0, yf_s1, yf_s2;   // yflux at row y-1
-    flux_y(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
+    flux_y<FAST>(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
 
     // software pipelining: the loads a row needs are issued one row ahead
     const int ylast = p.ny + 3;
@@ -356,9 +401,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
 
         // ---- faces of row y+1 along y: S face used now, N face kept
         Face S1, N1;
-        faces_1d(tt, eps, w_[2], w_[3], w_[4], hu_[1], hu_[2], hu_[3], hv_[1], hv_[2], hv_[3], BY_[1], BY_[2], S1, N1);
+        faces_1d<FAST>(tt, eps, w_[2], w_[3], w_[4], hu_[1], hu_[2], hu_[3], hv_[1], hv_[2], hv_[3], BY_[1], BY_[2], S1, N1);
         real yf0, yf1, yf2;   // yflux at row y
-        flux_y(g, half_g, Ncur, S1, yf0, yf1, yf2);
+        flux_y<FAST>(g, half_g, Ncur, S1, yf0, yf1, yf2);
 
         // ---- faces of row y along x
         const real wc = w_[2], huc = hu_[1], hvc = hv_[1];
@@ -367,14 +412,14 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         const real hvl = from_left(hvc), hvr = from_right(hvc);
         const real BXw = from_left(BXc);
         Face Wf, Ef;
-        faces_1d(tt, eps, wl, wc, wr, hul, huc, hur, hvl, hvc, hvr, BXw, BXc, Wf, Ef);
+        faces_1d<FAST>(tt, eps, wl, wc, wr, hul, huc, hur, hvl, hvc, hvr, BXw, BXc, Wf, Ef);
         // Pass1 stores u = (1/h)*hu and v = (1/h)*hv; along x the normal velocity is u.
         Face Wn;
         Wn.h = from_right(Wf.h);
         Wn.u = from_right(Wf.u);
         Wn.v = from_right(Wf.v);
         real xf0, xf1, xf2;   // xflux at (x, y)
-        flux_x(g, half_g, Ef, Wn, xf0, xf1, xf2);
+        flux_x<FAST>(g, half_g, Ef, Wn, xf0, xf1, xf2);
         const real xw0 = from_left(xf0), xw1 = from_left(xf1), xw2 = from_left(xf2);
 
         // ---- Pass3 at (x, y), compute.hlsl:616-812
@@ -413,16 +458,17 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             // here instead of read, which saves 24 bytes per cell of HBM traffic
             const real cs = p.coef_sl;
             real Kx, Qx, Ky, Qy;
-            line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal), fmaxr(0.f, cs - BAc),
+            line_kq<FAST>(p.kcoef, p.dx2, p.rdx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal), fmaxr(0.f, cs - BAc),
                     fmaxr(0.f, cs - bar), fmaxr(0.f, cs - barr), Kx, Qx);
-            line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - BA_[0]), fmaxr(0.f, cs - BA_[1]), fmaxr(0.f, cs - BA_[2]),
+            line_kq<FAST>(p.kcoef, p.dy2, p.rdy2, fmaxr(0.f, cs - BA_[0]), fmaxr(0.f, cs - BA_[1]), fmaxr(0.f, cs - BA_[2]),
                     fmaxr(0.f, cs - BA_[3]), fmaxr(0.f, cs - BA_[4]), Ky, Qy);
             const real d_l = fmaxr(0.f, sl - bal), d_r = fmaxr(0.f, sl - bar);
             const real d_ll = fmaxr(0.f, sl - ball), d_rr = fmaxr(0.f, sl - barr);
             const real d_d = fmaxr(0.f, sl - BA_[1]), d_u = fmaxr(0.f, sl - BA_[3]);
             const real d_dd = fmaxr(0.f, sl - BA_[0]), d_uu = fmaxr(0.f, sl - BA_[4]);
-            const real dd_by_dx = (-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * odx / 12.f;
-            const real dd_by_dy = (-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * ody / 12.f;
+            const real r12 = real(1) / real(12);
+            const real dd_by_dx = div_k<FAST>((-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * odx, real(12), r12);
+            const real dd_by_dy = div_k<FAST>((-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * ody, real(12), r12);
             const real eta_by_dx_dy = (0.25f * odx * ody * (eta_up_right - eta_down_right - eta_up_left + eta_down_left));
             const real eta_by_dx_dx = one_over_d2x * (eta_right - 2.f * eta_here + eta_left);
             const real eta_by_dy_dy = one_over_d2y * (eta_up - 2.f * eta_here + eta_down);
@@ -570,7 +616,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
 }
 
-template <int SCHEME, bool COMP, int R>
+template <int SCHEME, bool COMP, int R, bool FAST>
 __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
 {
     __shared__ ExplicitSmem<SCHEME == 2, R> sm;
@@ -580,7 +626,7 @@ __global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
     const int per_xcd = (ntiles + 7) / 8;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
-    explicit_tile<SCHEME, COMP, R>(p, tile, sm);
+    explicit_tile<SCHEME, COMP, R, SCHEME == 2, FAST>(p, tile, sm);
 }
 
 // ------------------------------------------------------------------ line solves
@@ -1365,8 +1411,18 @@ __global__ void transpose_plane(const real *in, int pin, real *out, int pout, in
     }
 }
 
+// 0, or a magnitude in [2^-80, 2^80]: the numerators div_k rounds correctly
+__device__ __forceinline__ bool div_k_range(real a)
+{
+    const real m = FABS(a);
+    return a == real(0) || (m >= real(0x1p-80f) && m <= real(0x1p80f));
+}
+
 // Line-coefficient fields of every interior cell from the bottom (engine.cpp:2656-2680: K1,
-// K2/72), row-major for the x-lines and transposed for the y-lines.
+// K2/72), row-major for the x-lines and transposed for the y-lines.  With p.unsafe set it
+// also checks the numerators of the explicit kernel's divisions by dx^2, dy^2, 72 and 12 at
+// the cell (K1's d^2, d, K2 of both lines, the depth-gradient sums) and raises *p.unsafe
+// when one lies outside div_k's range.
 __global__ void static_fields(const StaticParams p)
 {
     const int x = 2 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1375,14 +1431,26 @@ __global__ void static_fields(const StaticParams p)
     const real *BA = p.BA;
     auto ba = [&](int xx, int yy) { return BA[(size_t)yy * p.P + xx]; };
     const real cs = p.coef_sl;
+    auto dc = [&](int xx, int yy) { return fmaxr(0.f, cs - ba(xx, yy)); };
+    const real ex[5] = {dc(x - 2, y), dc(x - 1, y), dc(x, y), dc(x + 1, y), dc(x + 2, y)};
+    const real ey[5] = {dc(x, y - 2), dc(x, y - 1), ex[2], dc(x, y + 1), dc(x, y + 2)};
     real Kx, Qx, Ky, Qy;
-    line_kq(p.kcoef, p.dx2, fmaxr(0.f, cs - ba(x - 2, y)), fmaxr(0.f, cs - ba(x - 1, y)), fmaxr(0.f, cs - ba(x, y)),
-            fmaxr(0.f, cs - ba(x + 1, y)), fmaxr(0.f, cs - ba(x + 2, y)), Kx, Qx);
-    line_kq(p.kcoef, p.dy2, fmaxr(0.f, cs - ba(x, y - 2)), fmaxr(0.f, cs - ba(x, y - 1)), fmaxr(0.f, cs - ba(x, y)),
-            fmaxr(0.f, cs - ba(x, y + 1)), fmaxr(0.f, cs - ba(x, y + 2)), Ky, Qy);
+    line_kq(p.kcoef, p.dx2, p.rdx2, ex[0], ex[1], ex[2], ex[3], ex[4], Kx, Qx);
+    line_kq(p.kcoef, p.dy2, p.rdy2, ey[0], ey[1], ey[2], ey[3], ey[4], Ky, Qy);
     const size_t r = (size_t)y * p.P + (x - 2), c = (size_t)x * p.PT + (y - 2);
     p.Kx[r] = Kx; p.Qx[r] = Qx;
     p.KyT[c] = Ky; p.QyT[c] = Qy;
+    if (p.unsafe) {
+        const real d0 = ex[2];
+        const real K2x = (d0 / p.dx2) * (-ex[4] + 8.f * ex[3] - 8.f * ex[1] + ex[0]);
+        const real K2y = (d0 / p.dy2) * (-ey[4] + 8.f * ey[3] - 8.f * ey[1] + ey[0]);
+        auto ds = [&](int xx, int yy) { return fmaxr(0.f, p.sl - ba(xx, yy)); };
+        const real Nx = (-ds(x + 2, y) + 8.f * ds(x + 1, y) - 8.f * ds(x - 1, y) + ds(x - 2, y)) * p.odx;
+        const real Ny = (-ds(x, y + 2) + 8.f * ds(x, y + 1) - 8.f * ds(x, y - 1) + ds(x, y - 2)) * p.ody;
+        if (!(div_k_range(d0 * d0) && div_k_range(d0) && div_k_range(K2x) && div_k_range(K2y) && div_k_range(Nx) &&
+              div_k_range(Ny)))
+            atomicOr(p.unsafe, 1);
+    }
 }
 
 // level-0 coefficients of every cell for the x- and y-lines (diagnostic export): ghost rows
@@ -1418,10 +1486,14 @@ __global__ void export_coefficients(const StaticParams p, real *ax, real *ay)
 
 namespace CEL_NS {
 
-hipError_t launch_explicit(int scheme, bool comp, int rows, const ExplicitParams &p, hipStream_t st)
+hipError_t launch_explicit(int scheme, bool comp, int rows, bool fast, const ExplicitParams &p, hipStream_t st)
 {
     const dim3 grid(8 * ((p.nstrips * p.nbands + 7) / 8)), block(64);
-#define CEL_EXPLICIT(S, CP, R) hipLaunchKernelGGL((explicit_step<S, CP, R>), grid, block, 0, st, p)
+#define CEL_EXPLICIT(S, CP, R)                                                                  \
+    do {                                                                                        \
+        if (fast) hipLaunchKernelGGL((explicit_step<S, CP, R, true>), grid, block, 0, st, p);   \
+        else hipLaunchKernelGGL((explicit_step<S, CP, R, false>), grid, block, 0, st, p);       \
+    } while (0)
 #define CEL_EXPLICIT_R(S, CP)                                  \
     do {                                                       \
         if (rows == 4) CEL_EXPLICIT(S, CP, 4);                 \

@@ -70,6 +70,7 @@ struct ExplicitParams {
     // line-coefficient constants (BCOEF + 1/3, dx^2, dy^2, the coefficient sea level): the
     // kernel forms K1, K2/72 and the depth gradients from the bed as static_fields does
     real kcoef, dx2, dy2, coef_sl;
+    real rdx2, rdy2;               // RN(1/dx^2), RN(1/dy^2) for the FAST variants
 };
 
 // Static per-cell fields derived from the bottom once per mesh (bit-identical to computing
@@ -80,6 +81,9 @@ struct StaticParams {
     real *Kx, *Qx;                         // x-lines, row-major, [y * P + (x - 2)]
     real *KyT, *QyT;                       // y-lines, transposed, [x * PT + (y - 2)]
     real kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
+    real rdx2, rdy2;                       // RN(1/dx^2), RN(1/dy^2)
+    real sl, odx, ody;                     // sea level, 1/dx, 1/dy (the depth gradients)
+    int *unsafe;                           // raised when a cell rules out the FAST explicit kernel (or null)
 };
 
 struct LineParams {

@@ -321,6 +321,11 @@ typedef struct {
     int x_chunk, x_waves;       /* x-line solve: unknowns per thread (1..16), waves per line (1..8) */
     int y_chunk, y_waves;       /* y-line solve */
     float explicit_us, x_lines_us, y_lines_us;   /* timings behind an autotuned choice (else 0) */
+    /* Shortened exact division and square root in the explicit stencil (same bits as the
+     * library operations on the arguments it is used for).  set: 0 forbids it, nonzero lets
+     * the engine use it wherever the mesh and settings admit it (the default); get: 1 while
+     * it is in use. */
+    int fast_arith;
 } cel_tuning;
 
 /* Times every band height and every line-solve block shape that fits the engine's line

@@ -70,3 +70,57 @@ def test_autotuned_batch_bit_identical():
     for sid, sc in zip(ids, scs):
         assert np.array_equal(b.get_state_f64(sid), run(sc, 50))
     b.close()
+
+
+def shoreline(seed):
+    """A beach with wetting and drying and a soliton: divisions and roots see depths from
+    zero up."""
+    return S.small(nx=90, ny=64, seed=seed, soliton=True)
+
+
+@pytest.mark.parametrize("make", [lambda: case(100, 70, 47), lambda: case(120, 60, 48, comp=True),
+                                  lambda: shoreline(49)])
+def test_fast_arith_bit_identical(make):
+    """The explicit kernel's shortened exact division and square root give the bits of the
+    library operations: the engine that uses them equals one that is denied them."""
+    sc = make()
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    t = e.tuning()
+    assert t["fast_arith"] == 1
+    e.timestep(80)
+    a = e.get_state_f64()
+    e.close()
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.set_tuning(t["band_rows"], t["x_chunk"], t["x_waves"], t["y_chunk"], t["y_waves"], fast_arith=False)
+    assert e.tuning()["fast_arith"] == 0
+    e.timestep(80)
+    b = e.get_state_f64()
+    e.close()
+    assert np.array_equal(a, b)
+
+
+def test_fast_arith_ruled_out_by_mesh_and_settings():
+    """A depth of 1e-30 below the sea level (K1's d^2 underflows), an epsilon below 2^-96 or
+    a grid spacing below 2^-10 keep the plain kernel; a new bottom re-decides."""
+    sc = shoreline(50)
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    assert e.tuning()["fast_arith"] == 1
+    nb = sc.bottom.copy()
+    nb[2, 30, 40] = np.float32(sc.s.still_water_elevation) - np.float32(1e-30)
+    e.set_bottom(nb, keep_depth=True)
+    assert e.tuning()["fast_arith"] == 0
+    e.timestep(5)
+    e.set_bottom(sc.bottom, keep_depth=True)
+    assert e.tuning()["fast_arith"] == 1
+    e.close()
+    sc = shoreline(51)
+    sc.s.epsilon = 1e-30
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    assert e.tuning()["fast_arith"] == 0
+    e.close()
+    nx, ny = 40, 30
+    s = S.settings(nx, ny, 5e-4 * (nx - 1), 5e-4 * (ny - 1), 1e-5)
+    bottom = np.full((3, ny + 4, nx + 4), -0.01, np.float32)
+    e = ShallowWaterEngine(s, [Boundary(SOLID, 2)] * 4, bottom, S.initial_state(s, bottom, []))
+    assert e.tuning()["fast_arith"] == 0
+    e.close()
