@@ -114,9 +114,9 @@ void ShallowWaterEngine::allocate()
 {
     const size_t plane = (size_t)P_ * Y_ * sizeof(real);
     const size_t tplane = (size_t)PT_ * X_ * sizeof(real);
-    // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; Kx,Qx; compensated w_lo, hu_lo x2
-    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 2 + (comp_ ? 4 : 0);
-    const int ncol = 2 + 1 + 1 + 2 + (comp_ ? 2 : 0);         // hvT x2; DyT; ChvT; KyT, QyT; hvT_lo x2
+    // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; compensated w_lo, hu_lo x2
+    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + (comp_ ? 4 : 0);
+    const int ncol = 2 + 1 + 1 + 1 + (comp_ ? 2 : 0);         // hvT x2; DyT; ChvT; BAT; hvT_lo x2
     arena_bytes_ = plane * nrow + tplane * ncol;
     void *mem = nullptr;
     check(hipMalloc(&mem, arena_bytes_), "hipMalloc(state)");
@@ -132,8 +132,7 @@ void ShallowWaterEngine::allocate()
     for (auto &g : FG_) for (auto &c : g) c = rp();
     for (auto &c : FGp_) c = rp();
     Cw_ = rp(); Chu_ = rp();
-    Kx_ = rp(); Qx_ = rp();
-    hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); KyT_ = cp(); QyT_ = cp();
+    hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); BAT_ = cp();
     if (comp_) {
         w_lo_[0] = rp(); w_lo_[1] = rp(); hu_lo_[0] = rp(); hu_lo_[1] = rp();
         hvT_lo_[0] = cp(); hvT_lo_[1] = cp();
@@ -155,7 +154,7 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
     const size_t P = round_up((size_t)X + 16, 64), PT = round_up((size_t)Y + 16, 64);
     const bool comp = s.state_precision == CEL_STATE_COMPENSATED;
     const size_t plane = P * Y * sizeof(real), tplane = PT * X * sizeof(real);
-    size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + 2 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 2 + (comp ? 2 : 0));
+    size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 1 + (comp ? 2 : 0));
     const int tiles = ((4 * (s.nx / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (s.ny / 4) + kStatsTileY - 1) / kStatsTileY);
     b += ((size_t)tiles + 1) * kStatsValues * sizeof(double) + 256;
     for (int side = 0; side < 4; ++side) {
@@ -614,7 +613,7 @@ void ShallowWaterEngine::fillStatic(StaticParams &p) const
     std::memset(&p, 0, sizeof(p));
     p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
     p.BA = BA_;
-    p.Kx = Kx_; p.Qx = Qx_; p.KyT = KyT_; p.QyT = QyT_;
+    p.BAT = BAT_;
     p.kcoef = real(1) / real(15) + real(1) / real(3);
     p.dx2 = dx_ * dx_;
     p.dy2 = dy_ * dy_;
@@ -632,8 +631,16 @@ void ShallowWaterEngine::fillLines(LineParams &p) const
     std::memset(&p, 0, sizeof(p));
     p.nx = nx_; p.ny = ny_; p.P = P_; p.PT = PT_;
     p.nlines_x = ny_; p.nlines_y = nx_;
-    p.Dx = Dx_; p.Kx = Kx_; p.Qx = Qx_;
-    p.DyT = DyT_; p.KyT = KyT_; p.QyT = QyT_;
+    p.Dx = Dx_;
+    p.DyT = DyT_;
+    p.BA = BA_; p.BAT = BAT_;
+    p.kcoef = real(1) / real(15) + real(1) / real(3);     // as fillStatic
+    p.dx2 = dx_ * dx_;
+    p.dy2 = dy_ * dy_;
+    p.rdx2 = real(1) / p.dx2;
+    p.rdy2 = real(1) / p.dy2;
+    p.coef_sl = s_.is_boussinesq ? (real)s_.still_water_elevation : real(-9999);
+    p.fast = fastArith() ? 1 : 0;
     const int n = 1 - cur_;
     p.hu_out = hu_[n]; p.hvT_out = hvT_[n];
     const int c = cur_;

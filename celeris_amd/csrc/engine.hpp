@@ -177,7 +177,7 @@ private:
     bool comp_ = false;                                       // compensated two-float state
     real *w_lo_[2] = {}, *hu_lo_[2] = {}, *hvT_lo_[2] = {};
     real *BY_, *BX_, *BA_;
-    real *Kx_, *Qx_, *KyT_, *QyT_;      // line-coefficient fields (x row-major, y transposed)
+    real *BAT_;                         // BA transposed (the y-lines' coefficients)
     real *Dx_, *DyT_;
     real *G_[3][3];           // ring slot, component
     real *FG_[3][2];

@@ -453,9 +453,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             const real one_over_d2y = ody * ody;
             const real one_over_d3y = one_over_d2y * ody;
 
-            // K1, K2/72 of the x- and y-line through this cell and the fourth-order depth
-            // gradients, from the bed (the same operations as static_fields): recomputed
-            // here instead of read, which saves 24 bytes per cell of HBM traffic
+            // K1, K2/72 of the x- and y-line through this cell (the same operations as the
+            // line solves) and the fourth-order depth gradients, formed from the bed
+            // rather than read from stored planes
             const real cs = p.coef_sl;
             real Kx, Qx, Ky, Qy;
             line_kq<FAST>(p.kcoef, p.dx2, p.rdx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal), fmaxr(0.f, cs - BAc),
@@ -745,13 +745,59 @@ __device__ __forceinline__ void load_chunk(const real *src, int base, int n, rea
     }
 }
 
+// The bed along one line and the constants of its level-0 coefficients.
+struct LineBed {
+    const real *B;        // bed at ghost-inclusive position 0 of the line (16-byte aligned)
+    real kcoef, d2, rd2, cs;
+    bool fast;            // divisions through div_k
+};
+
+// The C + 4 bed values [base, base + C + 4) of a chunk (16-byte loads when the chunk is
+// whole; positions past the line's n + 4 read as 0 and are never used).
+template <int C>
+__device__ __forceinline__ void load_bed(const real *src, int base, int n, real *v)
+{
+    if ((C + 4) % kVec == 0 && C % kVec == 0 && base + C <= n) {
+#pragma unroll
+        for (int i = 0; i < C + 4; i += kVec) {
+            const Vec f = *reinterpret_cast<const Vec *>(src + base + i);
+#pragma unroll
+            for (int j = 0; j < kVec; ++j) v[i + j] = f.v[j];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < C + 4; ++i) v[i] = base + i < n + 4 ? src[base + i] : 0.f;
+    }
+}
+
+// Level-0 equations of a chunk: A, B, C from the depths at offsets -2..+2 (line_kq,
+// line_abc: the same operations as the explicit kernel), right-hand sides from dv.
+template <int C, bool FAST>
+__device__ __forceinline__ void chunk_equations(const LineBed &lb, const real *dd, const real *dv, int base, int n,
+                                                Eq *q)
+{
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int el = base + i;
+        if (el < n) {
+            real K1, Q;
+            line_kq<FAST>(lb.kcoef, lb.d2, lb.rd2, dd[i], dd[i + 1], dd[i + 2], dd[i + 3], dd[i + 4], K1, Q);
+            line_abc(K1, Q, el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
+            q[i].d = dv[i];
+        } else {
+            q[i] = Eq{0.f, 1.f, 0.f, 0.f};
+        }
+    }
+}
+
 // Line of n unknowns solved by a block of W waves; thread t owns positions [tC, tC+C).
-// D, K, Q point at element 0 of the line (16-byte aligned); out at element 0 of the
+// D points at element 0 of the line (16-byte aligned), lb.B at the line's ghost-inclusive
+// position 0 (element e's bed stencil is lb.B[e .. e+4]); out at element 0 of the
 // ghost-inclusive output line + 2 (8-byte aligned).
 // comp: the solution is an increment; out/out_lo receive the two-real sum of the previous
 // state (prev, prev_lo, same layout as out) and the increment.
 template <int C, int W>
-__device__ __forceinline__ void cr_line_block(int n, const real *D, const real *K, const real *Qf, real *out,
+__device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBed &lb, real *out,
                                               LineSmem<C, W> &sm, bool comp, const real *prev, const real *prev_lo,
                                               real *out_lo)
 {
@@ -763,20 +809,13 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const real *
     const int base = t * C;
     Eq q[C];
     {
-        real dv[C], kv[C], qv[C];
+        real dv[C], dd[C + 4];
         load_chunk<C>(D, base, n, dv);
-        load_chunk<C>(K, base, n, kv);
-        load_chunk<C>(Qf, base, n, qv);
+        load_bed<C>(lb.B, base, n, dd);
 #pragma unroll
-        for (int i = 0; i < C; ++i) {
-            const int el = base + i;
-            if (el < n) {
-                line_abc(kv[i], qv[i], el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
-                q[i].d = dv[i];
-            } else {
-                q[i] = Eq{0.f, 1.f, 0.f, 0.f};
-            }
-        }
+        for (int i = 0; i < C + 4; ++i) dd[i] = fmaxr(0.f, lb.cs - dd[i]);
+        if (lb.fast) chunk_equations<C, true>(lb, dd, dv, base, n, q);
+        else chunk_equations<C, false>(lb, dd, dv, base, n, q);
     }
     const Eq margin{0.f, 1.f, 0.f, 0.f};
     // ---- reduction inside the chunk: levels 1..K0
@@ -868,6 +907,18 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const real *
     }
 }
 
+__device__ __forceinline__ LineBed line_bed(const LineParams &p, bool xdir, int line)
+{
+    LineBed lb;
+    lb.B = xdir ? p.BA + (size_t)line * p.P : p.BAT + (size_t)line * p.PT;
+    lb.kcoef = p.kcoef;
+    lb.d2 = xdir ? p.dx2 : p.dy2;
+    lb.rd2 = xdir ? p.rdx2 : p.rdy2;
+    lb.cs = p.coef_sl;
+    lb.fast = p.fast != 0;
+    return lb;
+}
+
 // One block: an x-line, a y-line, or W irregular-wave lines.  (The body stays in the
 // kernel: routing the by-value parameter blocks through a device function makes the
 // compiler copy them to scratch.)
@@ -895,9 +946,10 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
     const int n = xdir ? p.nx : p.ny;
-    const real *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    const real *D = xdir ? p.Dx : p.DyT;
     real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
-    cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, COMP, COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
+    const LineBed lb = line_bed(p, xdir, line);
+    cr_line_block<C, W>(n, D + o, lb, out, sm, COMP, COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
                         COMP ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
                         COMP ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
 }
@@ -1116,10 +1168,11 @@ __global__ __launch_bounds__(64 * W) void solve_batched(const SolveJob *__restri
     const int line = xdir ? bid + 2 : (bid - p.nlines_x) + 2;
     const size_t o = xdir ? (size_t)line * p.P : (size_t)line * p.PT;
     const int n = xdir ? p.nx : p.ny;
-    const real *D = xdir ? p.Dx : p.DyT, *K = xdir ? p.Kx : p.KyT, *Q = xdir ? p.Qx : p.QyT;
+    const real *D = xdir ? p.Dx : p.DyT;
     real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
     const bool comp = J.comp != 0;
-    cr_line_block<C, W>(n, D + o, K + o, Q + o, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
+    const LineBed lb = line_bed(p, xdir, line);
+    cr_line_block<C, W>(n, D + o, lb, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
                         comp ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
                         comp ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
 }
@@ -1418,11 +1471,10 @@ __device__ __forceinline__ bool div_k_range(real a)
     return a == real(0) || (m >= real(0x1p-80f) && m <= real(0x1p80f));
 }
 
-// Line-coefficient fields of every interior cell from the bottom (engine.cpp:2656-2680: K1,
-// K2/72), row-major for the x-lines and transposed for the y-lines.  With p.unsafe set it
-// also checks the numerators of the explicit kernel's divisions by dx^2, dy^2, 72 and 12 at
-// the cell (K1's d^2, d, K2 of both lines, the depth-gradient sums) and raises *p.unsafe
-// when one lies outside div_k's range.
+// Range check of every interior cell for the FAST variants (explicit kernel and line
+// solves): the numerators of the divisions by dx^2, dy^2, 72 and 12 at the cell (K1's d^2,
+// d, K2 of both lines, the depth-gradient sums, engine.cpp:2656-2680 and compute.hlsl:
+// 641-697) must lie in div_k's range; *p.unsafe is raised otherwise.
 __global__ void static_fields(const StaticParams p)
 {
     const int x = 2 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1434,13 +1486,7 @@ __global__ void static_fields(const StaticParams p)
     auto dc = [&](int xx, int yy) { return fmaxr(0.f, cs - ba(xx, yy)); };
     const real ex[5] = {dc(x - 2, y), dc(x - 1, y), dc(x, y), dc(x + 1, y), dc(x + 2, y)};
     const real ey[5] = {dc(x, y - 2), dc(x, y - 1), ex[2], dc(x, y + 1), dc(x, y + 2)};
-    real Kx, Qx, Ky, Qy;
-    line_kq(p.kcoef, p.dx2, p.rdx2, ex[0], ex[1], ex[2], ex[3], ex[4], Kx, Qx);
-    line_kq(p.kcoef, p.dy2, p.rdy2, ey[0], ey[1], ey[2], ey[3], ey[4], Ky, Qy);
-    const size_t r = (size_t)y * p.P + (x - 2), c = (size_t)x * p.PT + (y - 2);
-    p.Kx[r] = Kx; p.Qx[r] = Qx;
-    p.KyT[c] = Ky; p.QyT[c] = Qy;
-    if (p.unsafe) {
+    {
         const real d0 = ex[2];
         const real K2x = (d0 / p.dx2) * (-ex[4] + 8.f * ex[3] - 8.f * ex[1] + ex[0]);
         const real K2y = (d0 / p.dy2) * (-ey[4] + 8.f * ey[3] - 8.f * ey[1] + ey[0]);
@@ -1463,18 +1509,19 @@ __global__ void export_coefficients(const StaticParams p, real *ax, real *ay)
     if (i >= X * Y) return;
     const int x = i % X, y = i / X;
     const size_t N = (size_t)X * Y;
-    real A = 0.f, B = 1.f, C = 0.f;
+    auto d = [&](int xx, int yy) { return fmaxr(0.f, p.coef_sl - p.BA[(size_t)yy * p.P + xx]); };
+    real A = 0.f, B = 1.f, C = 0.f, K1, Q;
     if (x >= 2 && x <= X - 3) {
         const int yy = y < 2 ? 2 : (y > p.ny + 1 ? p.ny + 1 : y);
-        const size_t r = (size_t)yy * p.P + (x - 2);
-        line_abc(p.Kx[r], p.Qx[r], x == 2, x == X - 3, A, B, C);
+        line_kq(p.kcoef, p.dx2, p.rdx2, d(x - 2, yy), d(x - 1, yy), d(x, yy), d(x + 1, yy), d(x + 2, yy), K1, Q);
+        line_abc(K1, Q, x == 2, x == X - 3, A, B, C);
     }
     ax[i] = A; ax[N + i] = B; ax[2 * N + i] = C;
     A = 0.f; B = 1.f; C = 0.f;
     if (y >= 2 && y <= Y - 3) {
         const int xx = x < 2 ? 2 : (x > p.nx + 1 ? p.nx + 1 : x);
-        const size_t c = (size_t)xx * p.PT + (y - 2);
-        line_abc(p.KyT[c], p.QyT[c], y == 2, y == Y - 3, A, B, C);
+        line_kq(p.kcoef, p.dy2, p.rdy2, d(xx, y - 2), d(xx, y - 1), d(xx, y), d(xx, y + 1), d(xx, y + 2), K1, Q);
+        line_abc(K1, Q, y == 2, y == Y - 3, A, B, C);
     }
     ay[i] = A; ay[N + i] = B; ay[2 * N + i] = C;
 }
@@ -1627,7 +1674,10 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
 
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st)
 {
-    hipLaunchKernelGGL(static_fields, dim3((p.nx + 127) / 128, p.ny), dim3(128), 0, st, p);
+    const int X = p.nx + 4, Y = p.ny + 4;
+    hipLaunchKernelGGL(transpose_plane, dim3((X + 31) / 32, (Y + 31) / 32), dim3(32, 8), 0, st, p.BA, p.P, p.BAT, p.PT,
+                       Y, X);
+    if (p.unsafe) hipLaunchKernelGGL(static_fields, dim3((p.nx + 127) / 128, p.ny), dim3(128), 0, st, p);
     return hipGetLastError();
 }
 

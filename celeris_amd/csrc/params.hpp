@@ -68,7 +68,7 @@ struct ExplicitParams {
     real ab[3];
     real der[9];
     // line-coefficient constants (BCOEF + 1/3, dx^2, dy^2, the coefficient sea level): the
-    // kernel forms K1, K2/72 and the depth gradients from the bed as static_fields does
+    // kernel forms K1, K2/72 and the depth gradients from the bed as the line solves do
     real kcoef, dx2, dy2, coef_sl;
     real rdx2, rdy2;               // RN(1/dx^2), RN(1/dy^2) for the FAST variants
 };
@@ -78,8 +78,7 @@ struct ExplicitParams {
 struct StaticParams {
     int nx, ny, P, PT;
     const real *BA;
-    real *Kx, *Qx;                         // x-lines, row-major, [y * P + (x - 2)]
-    real *KyT, *QyT;                       // y-lines, transposed, [x * PT + (y - 2)]
+    real *BAT;                             // BA transposed, [x * PT + y] (ghost-inclusive)
     real kcoef, dx2, dy2, coef_sl;         // (BCOEF + 1/3), dx^2, dy^2, coefficient sea level
     real rdx2, rdy2;                       // RN(1/dx^2), RN(1/dy^2)
     real sl, odx, ody;                     // sea level, 1/dx, 1/dy (the depth gradients)
@@ -89,8 +88,13 @@ struct StaticParams {
 struct LineParams {
     int nx, ny, P, PT;
     int nlines_x, nlines_y;         // blocks [0, nlines_x) solve x-lines, then y-lines
-    const real *Dx, *Kx, *Qx;      // x-lines: rhs and coefficient fields, [y * P + (x - 2)]
-    const real *DyT, *KyT, *QyT;   // y-lines: the same, transposed, [x * PT + (y - 2)]
+    const real *Dx;                // x-lines: right-hand sides, [y * P + (x - 2)]
+    const real *DyT;               // y-lines: the same, transposed, [x * PT + (y - 2)]
+    // the bed, from which each line forms its level-0 coefficients: row-major [y * P + x]
+    // for the x-lines and transposed [x * PT + y] for the y-lines (ghost-inclusive)
+    const real *BA, *BAT;
+    real kcoef, dx2, dy2, rdx2, rdy2, coef_sl;   // as ExplicitParams
+    int fast;                      // coefficient divisions through div_k (the mesh admits it)
     real *hu_out, *hvT_out;
     // compensated state: previous hi/lo words and the new low words (same layouts as out)
     const real *hu_prev, *hu_prev_lo, *hvT_prev, *hvT_prev_lo;
