@@ -175,6 +175,8 @@ void Batch::buildComposition(const std::vector<StepLaunch> &first, const std::ve
     st.push_back(0);
     for (int i = 0; i < ns; ++i) st.push_back(st.back() + cells[i]);
     c.total_cells = st.back();
+    c.one_pass = true;
+    for (const StepLaunch &L : first) c.one_pass = c.one_pass && L.bp.mirror_pass == 2;
     sync();
     clearGraphs();
     if (st.size() > starts_cap_) {
@@ -208,9 +210,13 @@ void Batch::issue(const char *blob, size_t stride, int m)
             if (comp_.cls_count[c] > 0)
                 check(launch_solve_batched(c, sj + comp_.cls_first[c], d_starts_ + comp_.cls_starts_off[c],
                                            comp_.cls_count[c], comp_.cls_blocks[c], stream_), "solve_batched");
-        for (int pass = 1; pass >= 0; --pass)
-            check(launch_boundary_batched(bj, d_starts_ + comp_.bnd_starts_off, ns, comp_.total_cells, pass, stream_),
+        if (comp_.one_pass)
+            check(launch_boundary_batched(bj, d_starts_ + comp_.bnd_starts_off, ns, comp_.total_cells, 2, stream_),
                   "boundary_batched");
+        else
+            for (int pass = 1; pass >= 0; --pass)
+                check(launch_boundary_batched(bj, d_starts_ + comp_.bnd_starts_off, ns, comp_.total_cells, pass,
+                                              stream_), "boundary_batched");
     }
 }
 

@@ -52,6 +52,7 @@ private:
         std::vector<int> solve_order;          // 2 i (x-lines) / 2 i + 1 (y-lines), sorted by block shape
         std::vector<int> sig;                  // per-scenario work counts and classes
         int total_tiles = 0, total_cells = 0;
+        bool one_pass = true;                  // every scenario's boundaries fit one launch (pass 2)
         int cls_first[kNumShapes] = {}, cls_count[kNumShapes] = {}, cls_blocks[kNumShapes] = {};
         int cls_starts_off[kNumShapes] = {};   // offsets into the starts buffer (ints)
         int exp_starts_off = 0, bnd_starts_off = 0;

@@ -717,6 +717,8 @@ void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
     bp.band_b = std::max(side_[CEL_SOUTH].cfg.width, 2);
     bp.band_t = std::max(side_[CEL_NORTH].cfg.width, 2);
     bp.ncells = (bp.band_l + bp.band_r) * Y_ + (X_ - bp.band_l - bp.band_r) * (bp.band_b + bp.band_t);
+    // one launch when no walk can pass both solid sides of an axis (boundary_cell, pass 2)
+    bp.mirror_pass = nx_ > std::max(bp.band_l, bp.band_r) + 2 && ny_ > std::max(bp.band_b, bp.band_t) + 2 ? 2 : 1;
     bp.t = total_time_;
     bp.w = w_[n]; bp.hu = hu_[n]; bp.hvT = hvT_[n];
     bp.w_lo = w_lo_[n]; bp.hu_lo = hu_lo_[n]; bp.hvT_lo = hvT_lo_[n];
@@ -783,10 +785,14 @@ void ShallowWaterEngine::issueStep(const StepLaunch &L, const real *dclock, bool
     if (profile) check(hipEventRecord(pe[2], stream_), "event");
     check(launch_solve(comp_, L.shape_x, L.shape_y, L.lp, irr, stream_), "solve_lines");
     if (profile) check(hipEventRecord(pe[3], stream_), "event");
-    bp.mirror_pass = 1;
-    check(launch_boundary(bp, stream_), "boundary_apply");
-    bp.mirror_pass = 0;
-    check(launch_boundary(bp, stream_), "boundary_apply");
+    if (bp.mirror_pass == 2) {
+        check(launch_boundary(bp, stream_), "boundary_apply");
+    } else {
+        bp.mirror_pass = 1;
+        check(launch_boundary(bp, stream_), "boundary_apply");
+        bp.mirror_pass = 0;
+        check(launch_boundary(bp, stream_), "boundary_apply");
+    }
     if (profile) {
         hipEvent_t e4 = nextEvent();
         check(hipEventRecord(e4, stream_), "event");

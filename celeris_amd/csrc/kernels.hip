@@ -991,59 +991,78 @@ __device__ __forceinline__ Val pre_value(const BoundaryParams &p, int x, int y)
     return v;
 }
 
-// Boundary cell tid of one scenario (tid < p.ncells).
-__device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, int mirror_pass)
+// The boundary cell set: left band, right band, then bottom/top bands between them (x
+// fastest inside each band: the row-major w and hu accesses of a wave share rows).
+__device__ __forceinline__ void boundary_cell_xy(const BoundaryParams &p, int tid, int &x, int &y)
 {
-    // enumerate the boundary cell set: left band, right band, then bottom/top bands between them
-    // (x fastest inside each band: the row-major w and hu accesses of a wave share rows)
-    int x, y;
-    {
-        int r = tid;
-        const int nl = p.band_l * p.Y, nr = p.band_r * p.Y;
-        const int mid = p.X - p.band_l - p.band_r;
-        if (r < nl) { y = r / p.band_l; x = r % p.band_l; }
-        else if ((r -= nl) < nr) { y = r / p.band_r; x = p.X - p.band_r + r % p.band_r; }
-        else {
-            r -= nr;
-            const int nb = p.band_b * mid;
-            if (r < nb) { y = r / mid; x = p.band_l + r % mid; }
-            else { r -= nb; y = p.Y - p.band_t + r / mid; x = p.band_l + r % mid; }
-        }
+    int r = tid;
+    const int nl = p.band_l * p.Y, nr = p.band_r * p.Y;
+    const int mid = p.X - p.band_l - p.band_r;
+    if (r < nl) { y = r / p.band_l; x = r % p.band_l; }
+    else if ((r -= nl) < nr) { y = r / p.band_r; x = p.X - p.band_r + r % p.band_r; }
+    else {
+        r -= nr;
+        const int nb = p.band_b * mid;
+        if (r < nb) { y = r / mid; x = p.band_l + r % mid; }
+        else { r -= nb; y = p.Y - p.band_t + r / mid; x = p.band_l + r % mid; }
     }
-    // walk the chain backwards through S, W, N, E (application order E, N, W, S); slot k
-    // records where side order[k] acts, so the unrolled loops index registers only
+}
+
+__device__ __forceinline__ bool in_boundary_set(const BoundaryParams &p, int x, int y)
+{
+    return x >= 0 && x < p.X && y >= 0 && y < p.Y &&
+           (x < p.band_l || x >= p.X - p.band_r || y < p.band_b || y >= p.Y - p.band_t);
+}
+
+// What the four strips (applied in the order E, N, W, S, each reading the state the
+// previous ones left) make of one cell: walking the order backwards from the cell, slot k
+// records where side order[k] acts; a solid side moves the walk to the mirrored cell, a
+// set-valued side (sine, irregular, time series) ends it.  The cell's final value is the
+// recorded sides applied in order to the pre-boundary value of the walk's end (cx, cy),
+// or to nothing when the walk ended at a set-valued side.
+struct Chain {
     int kx[4], ky[4];
     bool applied[4];
-    int cx = x, cy = y;
-    bool have_source = true, stop = false, mirrored = false;
+    int cx, cy;
+    bool have_source, mirrored;
+};
+
+__device__ __forceinline__ Chain boundary_chain(const BoundaryParams &p, int x, int y)
+{
+    Chain c;
+    c.cx = x; c.cy = y;
+    c.have_source = true; c.mirrored = false;
+    bool stop = false;
 #pragma unroll
     for (int k = 3; k >= 0; --k) {
         const int sd = k == 0 ? kEast : (k == 1 ? kNorth : (k == 2 ? kWest : kSouth));
-        applied[k] = !stop && in_strip(p, sd, cx, cy);
-        kx[k] = cx; ky[k] = cy;
-        if (applied[k]) {
+        c.applied[k] = !stop && in_strip(p, sd, c.cx, c.cy);
+        c.kx[k] = c.cx; c.ky[k] = c.cy;
+        if (c.applied[k]) {
             const int ty = p.side[sd].type;
             if (ty == kSolid) {
-                mirrored = true;
-                if (sd == kWest) cx = 4 - cx;
-                else if (sd == kSouth) cy = 4 - cy;
-                else if (sd == kEast) cx = (2 * p.nx + 2) - cx;
-                else cy = (2 * p.ny + 2) - cy;
+                c.mirrored = true;
+                if (sd == kWest) c.cx = 4 - c.cx;
+                else if (sd == kSouth) c.cy = 4 - c.cy;
+                else if (sd == kEast) c.cx = (2 * p.nx + 2) - c.cx;
+                else c.cy = (2 * p.ny + 2) - c.cy;
             } else if (ty != kSponge) {
-                have_source = false;
+                c.have_source = false;
                 stop = true;
             }
         }
     }
-    if ((int)mirrored != mirror_pass) return;
-    Val v{0.f, 0.f, 0.f};
-    if (have_source && cx >= 0 && cx < p.X && cy >= 0 && cy < p.Y) v = pre_value(p, cx, cy);
+    return c;
+}
+
+__device__ __forceinline__ Val apply_chain(const BoundaryParams &p, const Chain &c, Val v)
+{
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (!applied[k]) continue;
+        if (!c.applied[k]) continue;
         const int sd = k == 0 ? kEast : (k == 1 ? kNorth : (k == 2 ? kWest : kSouth));
         const SideParams &S = p.side[sd];
-        const int ox = kx[k], oy = ky[k];
+        const int ox = c.kx[k], oy = c.ky[k];
         switch (S.type) {
         case kSolid:
             if (sd == kWest || sd == kEast) v.hu = -v.hu; else v.hv = -v.hv;
@@ -1083,6 +1102,31 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, 
         }
         }
     }
+    return v;
+}
+
+// apply_chain for walks that end at a source cell: only solid and sponge sides lie on them
+__device__ __forceinline__ Val apply_chain_sourced(const BoundaryParams &p, const Chain &c, Val v)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!c.applied[k]) continue;
+        const int sd = k == 0 ? kEast : (k == 1 ? kNorth : (k == 2 ? kWest : kSouth));
+        const SideParams &S = p.side[sd];
+        if (S.type == kSolid) {
+            if (sd == kWest || sd == kEast) v.hu = -v.hu; else v.hv = -v.hv;
+        } else {
+            const real gamma = S.tab[strip_index(p, sd, c.kx[k], c.ky[k])];
+            v.w = gamma * (v.w - S.sl) + S.sl;
+            v.hu = gamma * v.hu;
+            v.hv = gamma * v.hv;
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ void write_boundary(const BoundaryParams &p, int x, int y, const Val &v)
+{
     p.w[(size_t)y * p.P + x] = v.w;
     p.hu[(size_t)y * p.P + x] = v.hu;
     p.hvT[(size_t)x * p.PT + y] = v.hv;
@@ -1090,6 +1134,57 @@ __device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, 
         p.w_lo[(size_t)y * p.P + x] = 0.f;
         p.hu_lo[(size_t)y * p.P + x] = 0.f;
         p.hvT_lo[(size_t)x * p.PT + y] = 0.f;
+    }
+}
+
+// Boundary cell tid of one scenario (tid < p.ncells).
+//   pass 1: cells whose chain passes a solid side; pass 0: all others (run after pass 1,
+//   so that the mirrored cells read their sources before those are rewritten in place).
+//   pass 2: every cell in one launch.  A cell whose walk ends at an interior cell of the
+//   boundary set (one this launch rewrites) is written by that cell's thread instead,
+//   which reads its own pre-boundary value before writing anything.  Such cells are
+//   mirror images of their source in x, y or both (at most one solid side per axis on the
+//   walk, which the host guarantees by the grid size).
+__device__ __forceinline__ void boundary_cell(const BoundaryParams &p, int tid, int pass)
+{
+    int x, y;
+    boundary_cell_xy(p, tid, x, y);
+    const Chain c = boundary_chain(p, x, y);
+    const bool interior = x >= 2 && x < p.nx + 2 && y >= 2 && y < p.ny + 2;
+    const Val own = pre_value(p, x, y);    // read before this thread writes anything
+    bool write_self = true;
+    if (pass != 2) {
+        if ((int)c.mirrored != pass) return;
+    } else if (c.have_source && (c.cx != x || c.cy != y) && c.cx >= 2 && c.cx < p.nx + 2 && c.cy >= 2 &&
+               c.cy < p.ny + 2 && in_boundary_set(p, c.cx, c.cy)) {
+        write_self = false;                // the thread of (c.cx, c.cy) writes this cell
+    }
+    if (write_self) {
+        Val v{0.f, 0.f, 0.f};
+        if (c.have_source && c.cx >= 0 && c.cx < p.X && c.cy >= 0 && c.cy < p.Y)
+            v = (c.cx == x && c.cy == y) ? own : pre_value(p, c.cx, c.cy);
+        write_boundary(p, x, y, apply_chain(p, c, v));
+    }
+    if (pass == 2 && interior) {
+        // the mirror images of this cell across solid sides whose walks end here
+        const bool mw = p.side[kWest].type == kSolid && 4 - x >= 0 && 4 - x < p.side[kWest].width;
+        const bool me = p.side[kEast].type == kSolid && (2 * p.nx + 2) - x >= p.X - p.side[kEast].width &&
+                        (2 * p.nx + 2) - x < p.X;
+        const bool ms = p.side[kSouth].type == kSolid && 4 - y >= 0 && 4 - y < p.side[kSouth].width;
+        const bool mn = p.side[kNorth].type == kSolid && (2 * p.ny + 2) - y >= p.Y - p.side[kNorth].width &&
+                        (2 * p.ny + 2) - y < p.Y;
+        if (mw || me || ms || mn) {
+#pragma unroll 1
+            for (int m = 1; m < 9; ++m) {
+                const int rx = m % 3, ry = m / 3;
+                if ((rx == 1 && !mw) || (rx == 2 && !me) || (ry == 1 && !ms) || (ry == 2 && !mn)) continue;
+                const int dx = rx == 0 ? x : (rx == 1 ? 4 - x : (2 * p.nx + 2) - x);
+                const int dy = ry == 0 ? y : (ry == 1 ? 4 - y : (2 * p.ny + 2) - y);
+                if (dx == x && dy == y) continue;
+                const Chain d = boundary_chain(p, dx, dy);
+                if (d.have_source && d.cx == x && d.cy == y) write_boundary(p, dx, dy, apply_chain_sourced(p, d, own));
+            }
+        }
     }
 }
 

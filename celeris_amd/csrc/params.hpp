@@ -149,7 +149,7 @@ struct BoundaryParams {
     int X, Y;
     int band_l, band_r, band_b, band_t;   // widths of the boundary cell set (>= 2 each)
     int ncells;
-    int mirror_pass;                      // 1: cells whose chain passes a solid side; 0: all others
+    int mirror_pass;                      // 1: cells whose chain passes a solid side; 0: all others; 2: all in one launch
     real t;
     const real *clock;                   // if set, t and the time-series values come from it
     real *w, *hu, *hvT;

@@ -63,6 +63,24 @@ def test_odd_sizes_sponge():
     check(sc, a, b, exact=True)
 
 
+def test_walls_meeting_sponges():
+    """Solid sides meeting sponges at every corner: mirrored ghost cells whose source is
+    rewritten by a sponge in the same boundary launch."""
+    bnd = [Boundary(SOLID, 2), Boundary(SPONGE, 6), Boundary(SPONGE, 5), Boundary(SOLID, 2)]
+    sc = S.small(nx=50, ny=40, seed=11, bnd=bnd)
+    a, b = run_pair(sc, 150)
+    check(sc, a, b, exact=True)
+
+
+def test_narrow_channel_two_boundary_passes():
+    """A channel narrower than two strips: the boundaries run as two launches (mirrored
+    cells, then the rest)."""
+    bnd = [Boundary(SOLID, 2), Boundary(SPONGE, 8), Boundary(SOLID, 2), Boundary(SPONGE, 6)]
+    sc = S.small(nx=9, ny=60, seed=12, bnd=bnd)
+    a, b = run_pair(sc, 150)
+    check(sc, a, b, exact=True)
+
+
 def test_wide_grid_multiple_strips_and_bands():
     sc = S.small(nx=157, ny=70, seed=5)
     a, b = run_pair(sc, 120)
