@@ -269,6 +269,7 @@ void Batch::fillTable(const std::vector<StepLaunch> &plans, const std::vector<Sh
             ej[i].p = L.ep;
             ej[i].scheme = L.scheme;
             ej[i].comp = eng[i]->compensated();
+            ej[i].fast = eng[i]->fastArith();
             bj[i].p = L.bp;
             bj[i].p.clock = nullptr;
         }

@@ -104,6 +104,8 @@ public:
     bool kernelTime(const std::string &name, float *ms, int *launches) override;
     size_t deviceBytes() const override { return arena_bytes_ + table_bytes_; }
     bool compensated() const { return comp_; }
+    // the explicit kernel's fast variant (shortened exact division and square root) applies
+    bool fastArith() const { return fast_mesh_ && fast_allowed_; }
     // device bytes an engine with these settings holds after creation (admission control)
     static size_t estimateBytes(const cel_settings &s, const cel_boundary bnd[4]);
     void coefficients(float *ax, float *ay) override;
@@ -211,7 +213,6 @@ private:
     int *d_flag_ = nullptr;
     bool fast_mesh_ = false, fast_allowed_ = true;
     bool fastSettings() const;
-    bool fastArith() const { return fast_mesh_ && fast_allowed_; }
 
     // scheme state (engine.cpp:3254-3269, 1476-1519)
     int cur_ = 0;

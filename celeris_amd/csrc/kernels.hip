@@ -1225,13 +1225,19 @@ __global__ __launch_bounds__(64) void explicit_batched(const ExplicitJob *__rest
     const int j = find_job(starts, njobs, t);
     const ExplicitJob &J = jobs[j];
     const int tile = t - starts[j];
-    switch (J.scheme * 2 + J.comp) {
-    case 0: explicit_tile<0, false, R>(J.p, tile, sm); break;
-    case 1: explicit_tile<0, true, R>(J.p, tile, sm); break;
-    case 2: explicit_tile<1, false, R>(J.p, tile, sm); break;
-    case 3: explicit_tile<1, true, R>(J.p, tile, sm); break;
-    case 6: explicit_tile<3, false, R>(J.p, tile, sm); break;
-    case 7: explicit_tile<3, true, R>(J.p, tile, sm); break;
+    switch (J.scheme * 4 + J.comp * 2 + J.fast) {
+    case 0: explicit_tile<0, false, R, false, false>(J.p, tile, sm); break;
+    case 1: explicit_tile<0, false, R, false, true>(J.p, tile, sm); break;
+    case 2: explicit_tile<0, true, R, false, false>(J.p, tile, sm); break;
+    case 3: explicit_tile<0, true, R, false, true>(J.p, tile, sm); break;
+    case 4: explicit_tile<1, false, R, false, false>(J.p, tile, sm); break;
+    case 5: explicit_tile<1, false, R, false, true>(J.p, tile, sm); break;
+    case 6: explicit_tile<1, true, R, false, false>(J.p, tile, sm); break;
+    case 7: explicit_tile<1, true, R, false, true>(J.p, tile, sm); break;
+    case 12: explicit_tile<3, false, R, false, false>(J.p, tile, sm); break;
+    case 13: explicit_tile<3, false, R, false, true>(J.p, tile, sm); break;
+    case 14: explicit_tile<3, true, R, false, false>(J.p, tile, sm); break;
+    case 15: explicit_tile<3, true, R, false, true>(J.p, tile, sm); break;
     default: break;   // corrector steps are not batched (rejected on the host)
     }
 }

@@ -160,7 +160,7 @@ struct BoundaryParams {
 // One scenario's part of a batched launch (parameters of one step).
 struct ExplicitJob {
     ExplicitParams p;
-    int scheme, comp;
+    int scheme, comp, fast;
 };
 struct SolveJob {
     LineParams p;
