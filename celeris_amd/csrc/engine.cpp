@@ -1117,12 +1117,13 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
     fillLines(lp);
     IrregularAll none;
     std::memset(&none, 0, sizeof(none));
+    float dir_us[2][kNumShapes];
     for (int dir = 0; dir < 2; ++dir) {
         const int n = dir == 0 ? nx_ : ny_;
         LineParams q = lp;
         if (dir == 0) q.nlines_y = 0; else q.nlines_x = 0;
-        float shape_us[kNumShapes];
-        for (auto &v : shape_us) v = -1.f;
+        float *shape_us = dir_us[dir];
+        for (int sh = 0; sh < kNumShapes; ++sh) shape_us[sh] = -1.f;
         for (int round = 0; round < kRounds; ++round)
             for (int sh = 0; sh < kNumShapes; ++sh) {
                 const int cap = shape_capacity(sh);
@@ -1136,6 +1137,28 @@ void ShallowWaterEngine::autotune(int repeats, cel_tuning *out)
             if (shape_us[sh] >= 0.f && (best_shape < 0 || shape_us[sh] < shape_us[best_shape])) best_shape = sh;
         if (dir == 0) shape_x_ = best_shape; else shape_y_ = best_shape;
         tuned_us_[1 + dir] = shape_us[best_shape];
+    }
+    // the step launches both directions at once (one launch when the shapes agree): time
+    // that launch for the pair chosen above and for every common shape, keep the fastest
+    {
+        const int nmax = std::max(nx_, ny_), nmin = std::min(nx_, ny_);
+        int best_x = shape_x_, best_y = shape_y_;
+        float best_us = -1.f;
+        for (int round = 0; round < kRounds; ++round)
+            for (int sh = -1; sh < kNumShapes; ++sh) {
+                int sx = shape_x_, sy = shape_y_;
+                if (sh >= 0) {
+                    const int cap = shape_capacity(sh);
+                    if (cap < nmax || cap > std::max(4 * nmin, 64)) continue;
+                    sx = sy = sh;
+                }
+                const float us = time_it([&] { check(launch_solve(comp_, sx, sy, lp, none, stream_), "solve_lines"); });
+                if (best_us < 0.f || us < best_us) { best_us = us; best_x = sx; best_y = sy; }
+            }
+        shape_x_ = best_x;
+        shape_y_ = best_y;
+        tuned_us_[1] = dir_us[0][shape_x_];
+        tuned_us_[2] = dir_us[1][shape_y_];
     }
     if (out) getTuning(out);
 }
