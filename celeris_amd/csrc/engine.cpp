@@ -181,7 +181,8 @@ void ShallowWaterEngine::uploadBottom(const float *bottom3)
     uploadHostBottom();
 }
 
-// Uploads hostBottom_ (BY, BX, BA planes) and recomputes the static per-cell fields.
+// Uploads hostBottom_ (BY, BX, BA planes), rebuilds the transposed BA and decides whether
+// the mesh admits the fast variants.
 void ShallowWaterEngine::uploadHostBottom()
 {
     const size_t N = (size_t)X_ * Y_;
@@ -1205,8 +1206,8 @@ void ShallowWaterEngine::uploadW(const std::vector<real> &w)
                            hipMemcpyHostToDevice, stream_), "upload w");
 }
 
-// After a bottom change: static fields, the depth-dependent boundary tables, no captured
-// blocks (their tables moved).
+// After a bottom change: the transposed bed and the fast-variant check, the
+// depth-dependent boundary tables, no captured blocks (their tables moved).
 void ShallowWaterEngine::bottomChanged()
 {
     clearGraphs();

@@ -260,7 +260,8 @@ __device__ __forceinline__ void line_abc(real K1, real Q, bool first, bool last,
 
 // ------------------------------------------------------------------ explicit step
 
-// Per-cell inputs of Pass3 that are read once: Adams-Bashforth history and static fields.
+// Per-cell inputs of Pass3 that are read once: the Adams-Bashforth history (and the low
+// word of w in the compensated state).
 struct CellIn {
     real o0, o1, o2, oo0, oo1, oo2, Fo, Go, Foo, Goo;
     real wlo;      // low word of w (compensated state)
