@@ -203,17 +203,18 @@ void ShallowWaterEngine::uploadHostBottom()
     fast_mesh_ = unsafe == 0 && fastSettings();
 }
 
-// The explicit kernel's FAST variant needs: FP32 state, epsilon >= 2^-96 (the CalcUV root
-// argument), dx^2 and dy^2 in [2^-20, 2^20], and every numerator of its divisions by dx^2,
-// dy^2, 72 and 12 in [2^-80, 2^80] or zero (checked per cell by static_fields).  Its
-// results are then bit-identical to the plain kernel's.
+// The explicit kernel's FAST variant needs: FP32 state, the Boussinesq coefficients (their
+// sea level is the still-water level, so the kernel forms each depth once), epsilon >=
+// 2^-96 (the CalcUV root argument), dx^2 and dy^2 in [2^-20, 2^20], and every numerator of
+// its divisions by dx^2, dy^2, 72 and 12 in [2^-80, 2^80] or zero (checked per cell by
+// static_fields).  Its results are then bit-identical to the plain kernel's.
 bool ShallowWaterEngine::fastSettings() const
 {
     if (sizeof(real) != 4) return false;
     ExplicitParams p;
     fillExplicit(p);
     auto in = [](real v) { return v >= real(0x1p-20f) && v <= real(0x1p20f); };
-    return p.epsilon >= real(0x1p-96f) && in(p.dx2) && in(p.dy2);
+    return p.coef_sl == p.sea_level && p.epsilon >= real(0x1p-96f) && in(p.dx2) && in(p.dy2);
 }
 
 // createSimTextures ring/scheme init, engine.cpp:3254-3269, and resetTimestep's first

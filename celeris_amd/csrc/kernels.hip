@@ -53,6 +53,10 @@ __device__ __forceinline__ double POW(double x, double y) { return pow(x, y); }
 //     by the sign of the residual on either neighbour, as the compiler's own expansion of
 //     sqrtf does for arguments that need no rescaling; the rescaling and the special-value
 //     tests it adds for the others are dropped.
+//   sqrt_ks(x): any x that is not NaN.  The compiler's expansion with its rescaling of
+//     x < 2^-96 (by 2^32, the root by 2^-16) but without its final test for +-0 and +inf:
+//     the corrected root already returns x there (the neighbour residuals come out NaN or
+//     zero and select nothing).
 template <bool FAST>
 __device__ __forceinline__ real div_k(real a, real b, real rb)
 {
@@ -64,6 +68,23 @@ __device__ __forceinline__ real div_k(real a, real b, real rb)
 #endif
     (void)rb;
     return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ real sqrt_ks(real x)
+{
+#ifndef CEL_FP64
+    if (FAST) {
+        const bool small = x < 0x1p-96f;
+        const float xs = small ? x * 0x1p32f : x;
+        const float s = __builtin_amdgcn_sqrtf(xs);
+        const float sm = __int_as_float(__float_as_int(s) - 1);
+        const float sp = __int_as_float(__float_as_int(s) + 1);
+        float r = __builtin_fmaf(-sm, s, xs) <= 0.f ? sm : s;
+        r = __builtin_fmaf(-sp, s, xs) > 0.f ? sp : r;
+        return small ? r * 0x1p-16f : r;
+    }
+#endif
+    return SQRT(x);
 }
 template <bool FAST>
 __device__ __forceinline__ real sqrt_k(real x)
@@ -218,8 +239,8 @@ __device__ __forceinline__ void faces_1d(real two_theta, real eps,
 template <bool FAST>
 __device__ __forceinline__ void flux_x(real g, real half_g, const Face &E, const Face &Wn, real &f0, real &f1, real &f2)
 {
-    const real cE = SQRT(g * E.h);
-    const real cW = SQRT(g * Wn.h);
+    const real cE = sqrt_ks<FAST>(g * E.h);
+    const real cW = sqrt_ks<FAST>(g * Wn.h);
     const real ap = fmaxr(fmaxr(E.u + cE, Wn.u + cW), 0.f);
     const real am = fminr(fminr(E.u - cE, Wn.u - cW), 0.f);
     f0 = num_flux(ap, am, Wn.h * Wn.u, E.h * E.u, Wn.h - E.h);
@@ -231,8 +252,8 @@ __device__ __forceinline__ void flux_x(real g, real half_g, const Face &E, const
 template <bool FAST>
 __device__ __forceinline__ void flux_y(real g, real half_g, const Face &N, const Face &Sn, real &f0, real &f1, real &f2)
 {
-    const real cN = SQRT(g * N.h);
-    const real cS = SQRT(g * Sn.h);
+    const real cN = sqrt_ks<FAST>(g * N.h);
+    const real cS = sqrt_ks<FAST>(g * Sn.h);
     const real bp = fmaxr(fmaxr(N.v + cN, Sn.v + cS), 0.f);
     const real bm = fminr(fminr(N.v - cN, Sn.v - cS), 0.f);
     f0 = num_flux(bp, bm, Sn.h * Sn.v, N.h * N.v, Sn.h - N.h);
@@ -340,6 +361,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
 
     // register windows over rows; index 0 = y-2 ... 4 = y+2
     real w_[5], BA_[5], hu_[4], hv_[4], BY_[4];
+    // fast variant (coefficient sea level == still-water level): the depths max(0, sl - BA)
+    // of rows y-2 .. y+2, formed once per row and shared by K1, K2/72, the gradients and Psi
+    real D_[5];
     // window slot j holds row (y - 2 + j) for w/BA; (y - 1 + j) for hu/hv/BY
     // 32-bit element offsets: planes are far below 4 G elements, and a uniform base with a
     // 32-bit offset maps to the scalar-base addressing mode (no 64-bit address arithmetic)
@@ -350,6 +374,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         const int row = ys - 2 + j;
         w_[j] = ldrow(p.w, row);
         BA_[j] = ldrow(p.BA, row);
+        if (FAST) D_[j] = fmaxr(0.f, sl - BA_[j]);
     }
     for (int j = 0; j < 3; ++j) {
         const int row = ys - 1 + j;
@@ -388,6 +413,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         BY_[3] = nx_BY;
         const real BXc = nx_BX;
         BA_[4] = nx_BA;
+        if (FAST) D_[4] = fmaxr(0.f, sl - BA_[4]);
         const real BAc = BA_[2];
         const CellIn cin = nx_c;
         {
@@ -430,13 +456,14 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         const real BYs = BY_[0];
         const real BYc = BY_[1];
         // the bed at x-2 .. x+2 for the line coefficients and the depth gradients
-        const real bal = from_left(BAc), bar = from_right(BAc);
+        // (fast variant: the depths at x-2 .. x+2 instead)
+        const real bal = from_left(FAST ? D_[2] : BAc), bar = from_right(FAST ? D_[2] : BAc);
         const real ball = from_left(bal), barr = from_right(bar);
 
         if (out_lane) {
             const unsigned i = (unsigned)y * (unsigned)P + (unsigned)x;
             const real h = fmaxr(0.f, wc - BAc);
-            const real d_here = fmaxr(0.f, sl - BAc);
+            const real d_here = FAST ? D_[2] : fmaxr(0.f, sl - BAc);
             const real d2_here = d_here * d_here;
             const real d3_here = d2_here * d_here;
 
@@ -459,14 +486,19 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             // rather than read from stored planes
             const real cs = p.coef_sl;
             real Kx, Qx, Ky, Qy;
-            line_kq<FAST>(p.kcoef, p.dx2, p.rdx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal), fmaxr(0.f, cs - BAc),
-                    fmaxr(0.f, cs - bar), fmaxr(0.f, cs - barr), Kx, Qx);
-            line_kq<FAST>(p.kcoef, p.dy2, p.rdy2, fmaxr(0.f, cs - BA_[0]), fmaxr(0.f, cs - BA_[1]), fmaxr(0.f, cs - BA_[2]),
-                    fmaxr(0.f, cs - BA_[3]), fmaxr(0.f, cs - BA_[4]), Ky, Qy);
-            const real d_l = fmaxr(0.f, sl - bal), d_r = fmaxr(0.f, sl - bar);
-            const real d_ll = fmaxr(0.f, sl - ball), d_rr = fmaxr(0.f, sl - barr);
-            const real d_d = fmaxr(0.f, sl - BA_[1]), d_u = fmaxr(0.f, sl - BA_[3]);
-            const real d_dd = fmaxr(0.f, sl - BA_[0]), d_uu = fmaxr(0.f, sl - BA_[4]);
+            if (FAST) {
+                line_kq<FAST>(p.kcoef, p.dx2, p.rdx2, ball, bal, D_[2], bar, barr, Kx, Qx);
+                line_kq<FAST>(p.kcoef, p.dy2, p.rdy2, D_[0], D_[1], D_[2], D_[3], D_[4], Ky, Qy);
+            } else {
+                line_kq<FAST>(p.kcoef, p.dx2, p.rdx2, fmaxr(0.f, cs - ball), fmaxr(0.f, cs - bal),
+                              fmaxr(0.f, cs - BAc), fmaxr(0.f, cs - bar), fmaxr(0.f, cs - barr), Kx, Qx);
+                line_kq<FAST>(p.kcoef, p.dy2, p.rdy2, fmaxr(0.f, cs - BA_[0]), fmaxr(0.f, cs - BA_[1]),
+                              fmaxr(0.f, cs - BA_[2]), fmaxr(0.f, cs - BA_[3]), fmaxr(0.f, cs - BA_[4]), Ky, Qy);
+            }
+            const real d_l = FAST ? bal : fmaxr(0.f, sl - bal), d_r = FAST ? bar : fmaxr(0.f, sl - bar);
+            const real d_ll = FAST ? ball : fmaxr(0.f, sl - ball), d_rr = FAST ? barr : fmaxr(0.f, sl - barr);
+            const real d_d = FAST ? D_[1] : fmaxr(0.f, sl - BA_[1]), d_u = FAST ? D_[3] : fmaxr(0.f, sl - BA_[3]);
+            const real d_dd = FAST ? D_[0] : fmaxr(0.f, sl - BA_[0]), d_uu = FAST ? D_[4] : fmaxr(0.f, sl - BA_[4]);
             const real r12 = real(1) / real(12);
             const real dd_by_dx = div_k<FAST>((-d_rr + 8.f * d_r - 8.f * d_l + d_ll) * odx, real(12), r12);
             const real dd_by_dy = div_k<FAST>((-d_uu + 8.f * d_u - 8.f * d_d + d_dd) * ody, real(12), r12);
@@ -593,6 +625,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         // ---- shift windows
         w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];
         BA_[0] = BA_[1]; BA_[1] = BA_[2]; BA_[2] = BA_[3]; BA_[3] = BA_[4];
+        if (FAST) { D_[0] = D_[1]; D_[1] = D_[2]; D_[2] = D_[3]; D_[3] = D_[4]; }
         hu_[0] = hu_[1]; hu_[1] = hu_[2]; hu_[2] = hu_[3];
         hv_[0] = hv_[1]; hv_[1] = hv_[2]; hv_[2] = hv_[3];
         BY_[0] = BY_[1]; BY_[1] = BY_[2]; BY_[2] = BY_[3];
