@@ -870,29 +870,33 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
             q[pp] = reduce_eq(L, q[pp], R);
         }
     }
-    // ---- reduction across threads: levels K0+1 .. K0+KT, in LDS
-    const int mypos = base + C - 1;
+    // ---- reduction across threads: levels K0+1 .. K0+KT, in LDS.  Slot i holds the chunk
+    // top of thread i (line position iC + C - 1).  Level j updates slots (m+1) 2^j - 1; they
+    // are dealt to threads m = 0, 1, ... so that the waves with no slot skip the level.
     sm.set_top(t, q[C - 1]);
     __syncthreads();
 #pragma unroll
     for (int j = 1; j <= KT; ++j) {
         const int s = 1 << (j - 1);
-        if (((t + 1) & ((1 << j) - 1)) == 0 && mypos < n) {
-            const Eq L = sm.top(t - s);
-            const Eq R = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.top(t + s) : margin;
-            sm.set_top(t, reduce_eq(L, sm.top(t), R));
+        const int i = ((t + 1) << j) - 1;
+        if (i < NT && i * C + C - 1 < n) {
+            const Eq L = sm.top(i - s);
+            const Eq R = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.top(i + s) : margin;
+            sm.set_top(i, reduce_eq(L, sm.top(i), R));
         }
         __syncthreads();
     }
-    // ---- back-substitution across threads: levels K0+KT .. K0 (solution in .d)
+    // ---- back-substitution across threads: levels K0+KT .. K0 (solution in .d); level j
+    // solves slots (2m+1) 2^j - 1, dealt to threads m in the same way
 #pragma unroll
     for (int j = KT; j >= 0; --j) {
         const int s = 1 << j;
-        if (((t + 1) & ((2 << j) - 1)) == s && mypos < n) {
-            const real xl = t - s >= 0 ? sm.td[t - s] : 0.f;
-            const real xr = (t + s < NT && (t + s) * C + C - 1 < n) ? sm.td[t + s] : 0.f;
-            const Eq e = sm.top(t);
-            sm.td[t] = (e.d - e.a * xl - e.c * xr) / e.b;
+        const int i = ((2 * t + 1) << j) - 1;
+        if (i < NT && i * C + C - 1 < n) {
+            const real xl = i - s >= 0 ? sm.td[i - s] : 0.f;
+            const real xr = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.td[i + s] : 0.f;
+            const Eq e = sm.top(i);
+            sm.td[i] = (e.d - e.a * xl - e.c * xr) / e.b;
         }
         __syncthreads();
     }
