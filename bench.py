@@ -41,12 +41,17 @@ HBM_PEAK_GBS = 8000.0             # MI355X HBM3E peak
 PMC_FILE = os.path.join(ROOT, "profiles", "explicit_step_2048x2048_pmc.json")
 
 
-def measured_traffic(nx, ny):
+def measured_traffic(nx, ny, band_rows):
+    """HBM bytes per launch and VALU utilisation of the explicit kernel variant with this band
+    height, from its rocprofv3 counter runs (None where none was profiled)."""
     if (nx, ny) != (2048, 2048) or not os.path.exists(PMC_FILE):
         return None, None
     with open(PMC_FILE) as f:
         d = json.load(f)
-    return d["traffic_bytes_per_launch"], d.get("valu_busy")
+    v = d.get("by_band_rows", {}).get(str(band_rows))
+    if v is None:
+        return None, None
+    return v["traffic_bytes_per_launch"], v.get("valu_busy")
 
 
 def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
@@ -201,6 +206,8 @@ def main():
                          "MI355X; the ensemble workload defaults to 50-step blocks)")
     ap.add_argument("--autotune", type=int, default=10,
                     help="repeats per candidate when autotuning band height and line-solve shapes (0: defaults)")
+    ap.add_argument("--band-rows", type=int, default=0, choices=[0, 4, 8, 12, 16],
+                    help="pin the explicit kernel's band height after autotuning (0: keep the autotuned one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
@@ -257,6 +264,9 @@ def main():
     eng.set_graph_steps(args.graph_steps)
     eng.timestep(args.warmup)
     tuning = eng.autotune(args.autotune) if args.autotune > 0 else eng.tuning()
+    if args.band_rows:
+        eng.set_tuning(args.band_rows, tuning["x_chunk"], tuning["x_waves"], tuning["y_chunk"], tuning["y_waves"])
+        tuning = eng.tuning()
     eng.timestep(10)
     eng.sync()
 
@@ -280,7 +290,8 @@ def main():
 
     avg_ms = kern["explicit_step"]
     achieved = bytes_per_cell * cells / (avg_ms * 1e-3) / 1e9
-    traffic, valu_busy = measured_traffic(sc.s.nx, sc.s.ny) if args.precision == "fp32" else (None, None)
+    traffic, valu_busy = (measured_traffic(sc.s.nx, sc.s.ny, tuning["band_rows"]) if args.precision == "fp32"
+                          else (None, None))
     out = {
         "metric": "Gcell-steps/s (2048^2 Boussinesq)",
         "value": value,
@@ -303,7 +314,9 @@ def main():
                    "event_ms_per_step": ms_events / args.steps,
                    "kernel_avg_ms": kern, "state_finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "explicit_step",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"explicit_step (AB3, {tuning['band_rows']}-row bands"
+                               + (", fast variant)" if tuning.get("fast_arith") else ")"),
                      "bytes_per_launch": bytes_per_cell * cells,
                      "measured_hbm_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9,
                      "valu_busy": valu_busy},
