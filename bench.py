@@ -74,6 +74,36 @@ def cpu_baseline(sc, seconds_target=15.0, max_steps=40):
                       f"{dt:.1f} s, OpenMP {threads} threads"}
 
 
+def cpu_baselines(seconds_target=10.0):
+    """The CPU oracles (FP32, FP64) single-threaded and with OpenMP on the host's cores, on the
+    plain 256^2 beach case and the 2048^2 headline case.  Each run times as many steps as fit
+    about seconds_target after 2 warm-up steps (the 256^2 case at most its full 2000 steps)."""
+    from oracle.binding import Oracle, get_threads, set_threads
+    from celeris_amd import scenarios as S
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rows = []
+    for name, sc, cap in (("plain 256x256", S.plain_beach(), 2000), ("headline 2048x2048", S.headline(), 100)):
+        for prec in ("f32", "f64"):
+            for threads in (1, all_threads):
+                set_threads(threads)
+                o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, prec)
+                t0 = time.perf_counter()
+                o.step(2)
+                per = max((time.perf_counter() - t0) / 2, 1e-6)
+                n = int(max(3, min(cap, seconds_target / per)))
+                t0 = time.perf_counter()
+                o.step(n)
+                dt = time.perf_counter() - t0
+                cells = sc.s.nx * sc.s.ny
+                rows.append({"case": name, "precision": prec, "threads": get_threads(), "steps": n,
+                             "seconds": round(dt, 2), "gcell_steps_per_s": cells * n / dt / 1e9})
+                print(f"bench: cpu {name} {prec} {threads} threads: {rows[-1]['gcell_steps_per_s']:.4f} "
+                      f"Gcell-steps/s ({n} steps)", file=sys.stderr, flush=True)
+                del o
+    set_threads(all_threads)
+    return rows
+
+
 def serving_leg(n, rates, seed, block):
     """A bounded sample of the serving stream: n requests (generated before the clock
     starts), served in real time with continuous batching at each arrival rate, and the
@@ -211,7 +241,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
-    ap.add_argument("--workload", choices=["headline", "ensemble", "precision"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
@@ -246,6 +276,15 @@ def main():
         if "--nx" not in sys.argv:
             args.nx, args.ny = 4096, 1024
         print(json.dumps(precision_line(args)), flush=True)
+        return
+    if args.workload == "cpu":
+        rows = cpu_baselines()
+        head = [r for r in rows if r["case"].startswith("headline") and r["precision"] == "f32"][-1]
+        print(json.dumps({"metric": "CPU oracle baselines (reported only)", "value": head["gcell_steps_per_s"],
+                          "unit": "Gcell-steps/s", "n_gpus": 0, "steps": head["steps"], "warmup": 2,
+                          "higher_is_better": True, "vs_baseline": None, "dtype": "f32 / f64", "data": "synthetic",
+                          "config": {"workload": "CPU FP32 and FP64 oracles, 1 thread and OpenMP, plain 256x256 "
+                                                 "beach and 2048x2048 headline", "runs": rows}}), flush=True)
         return
     if args.workload == "ensemble":
         if args.graph_steps == 0 and "--graph-steps" not in sys.argv:

@@ -69,6 +69,8 @@ def lib():
             getattr(L, f"orc_{p}_edit_water").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 5 + [ctypes.c_int]
             getattr(L, f"orc_{p}_get_bottom").argtypes = [ctypes.c_void_p, R]
         L.orc_bottom_from_corners.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_f32_minmod.restype = ctypes.c_float
         L.orc_f32_minmod.argtypes = [ctypes.c_float] * 3
         L.orc_f32_reconstruct.argtypes = [ctypes.c_float] * 4 + [_F, _F]
@@ -227,3 +229,12 @@ def bottom_from_corners(nx, ny, z):
     out = np.zeros((3, ny + 4, nx + 4), np.float32)
     lib().orc_bottom_from_corners(nx, ny, fptr(z), fptr(out))
     return out
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's row loops (timing only; results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())

@@ -60,6 +60,10 @@ typedef struct {
     orc_boundary bnd[4];              /* indexed by ORC_WEST..ORC_NORTH */
 } orc_config;
 
+/* OpenMP threads of the row loops (timing only). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 typedef struct orc_sim_f32 orc_sim_f32;
 typedef struct orc_sim_f64 orc_sim_f64;
 

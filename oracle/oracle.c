@@ -7,6 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include "celeris_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ---- FP32 instance ---- */
 typedef float real;
@@ -53,6 +56,24 @@ typedef float real;
 #define EXP exp
 #include "oracle_impl.inc"
 #undef real
+
+/* OpenMP threads of the oracle's row loops (timing only; results do not depend on it). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* UpdateTerrainHeightfield, terrain_heightfield.cpp:244-298: corner heights at
  * ((i-2.5), (j-2.5)) for i<nx+5, j<ny+5 reduced to the per-cell face/centre values
