@@ -370,8 +370,70 @@ int Batch::autotune(int repeats, bool tune_lines)
         if (cand_ms[c] < best) { best = cand_ms[c]; best_rows = cand[c]; }
     rows_ = best_rows;
     comp_valid_ = false;
+    if (tune_lines) tuneSharedLineShape(repeats);
+    comp_valid_ = false;
     blob_used_[0] = blob_used_[1] = false;
     return rows_;
+}
+
+// The line solves run one launch per block-shape class.  Each member's own best shapes can
+// leave several short launches whose tails idle the GPU, so the same lines under one shape
+// shared by every member (a single launch) are timed too, and the faster plan is kept.
+void Batch::tuneSharedLineShape(int repeats)
+{
+    std::vector<ShallowWaterEngine *> act;
+    int nmax = 0;
+    for (auto &kv : scen_)
+        if (!paused_.count(kv.first)) {
+            act.push_back(kv.second.get());
+            nmax = std::max(nmax, kv.second->lineLength());
+        }
+    if (act.empty()) return;
+    std::vector<std::pair<int, int>> own;
+    for (auto *e : act) own.emplace_back(e->shapeX(), e->shapeY());
+    auto time_solves = [&]() {
+        std::vector<StepLaunch> plans;
+        std::vector<ShallowWaterEngine *> eng;
+        comp_valid_ = false;
+        prepare(1, false, plans, eng);
+        sync();
+        char *h = h_blob_[0];
+        fillTable(plans, eng, 1, h);
+        check(hipMemcpy(d_blob_, h, stride_, hipMemcpyHostToDevice), "upload batch table");
+        const SolveJob *sj = reinterpret_cast<const SolveJob *>(d_blob_ + sec_solve_);
+        auto launch = [&] {
+            for (int c = 0; c < kNumShapes; ++c)
+                if (comp_.cls_count[c] > 0)
+                    check(launch_solve_batched(c, sj + comp_.cls_first[c], d_starts_ + comp_.cls_starts_off[c],
+                                               comp_.cls_count[c], comp_.cls_blocks[c], stream_), "solve_batched");
+        };
+        launch();
+        check(hipEventRecord(ev_[0], stream_), "event");
+        for (int r = 0; r < repeats; ++r) launch();
+        check(hipEventRecord(ev_[1], stream_), "event");
+        check(hipEventSynchronize(ev_[1]), "event sync");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event");
+        return ms;
+    };
+    std::vector<int> shapes = {-1};            // -1: every member's own shapes
+    for (int sh = 0; sh < kNumShapes; ++sh)
+        if (shape_capacity(sh) >= nmax && shape_capacity(sh) <= 4 * nmax) shapes.push_back(sh);
+    std::vector<float> ms(shapes.size(), -1.f);
+    for (int round = 0; round < 3; ++round)
+        for (size_t c = 0; c < shapes.size(); ++c) {
+            for (size_t i = 0; i < act.size(); ++i)
+                if (shapes[c] < 0) act[i]->setLineShapes(own[i].first, own[i].second);
+                else act[i]->setLineShapes(shapes[c], shapes[c]);
+            const float t = time_solves();
+            if (ms[c] < 0.f || t < ms[c]) ms[c] = t;
+        }
+    size_t best = 0;
+    for (size_t c = 1; c < shapes.size(); ++c)
+        if (ms[c] < ms[best]) best = c;
+    for (size_t i = 0; i < act.size(); ++i)
+        if (shapes[best] < 0) act[i]->setLineShapes(own[i].first, own[i].second);
+        else act[i]->setLineShapes(shapes[best], shapes[best]);
 }
 
 void Batch::step(int nsteps)

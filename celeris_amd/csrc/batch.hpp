@@ -63,6 +63,7 @@ private:
     void prepare(int m, bool advance, std::vector<StepLaunch> &plans, std::vector<ShallowWaterEngine *> &eng);
     void fillTable(const std::vector<StepLaunch> &plans, const std::vector<ShallowWaterEngine *> &eng, int m, char *h);
     void buildComposition(const std::vector<StepLaunch> &first, const std::vector<int> &ids);
+    void tuneSharedLineShape(int repeats);
     void issue(const char *blob, size_t stride, int m);
     void clearGraphs();
     void check(hipError_t e, const char *what) const;

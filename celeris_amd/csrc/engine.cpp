@@ -210,11 +210,11 @@ void ShallowWaterEngine::uploadHostBottom()
 // static_fields).  Its results are then bit-identical to the plain kernel's.
 bool ShallowWaterEngine::fastSettings() const
 {
-    if (sizeof(real) != 4) return false;
-    ExplicitParams p;
-    fillExplicit(p);
+    if (sizeof(real) != 4 || !s_.is_boussinesq) return false;      // coefficient sea level == SWL
+    const real m = std::max(dx_, dy_);
+    const real eps = std::min((real)s_.epsilon, (m * m) * (m * m));   // as fillExplicit
     auto in = [](real v) { return v >= real(0x1p-20f) && v <= real(0x1p20f); };
-    return p.coef_sl == p.sea_level && p.epsilon >= real(0x1p-96f) && in(p.dx2) && in(p.dy2);
+    return eps >= real(0x1p-96f) && in(dx_ * dx_) && in(dy_ * dy_);
 }
 
 // createSimTextures ring/scheme init, engine.cpp:3254-3269, and resetTimestep's first
@@ -1075,6 +1075,16 @@ void ShallowWaterEngine::setTuning(const cel_tuning &t)
     shape_y_ = sy;
     fast_allowed_ = t.fast_arith != 0;
     tuned_us_[0] = tuned_us_[1] = tuned_us_[2] = 0.f;
+}
+
+void ShallowWaterEngine::setLineShapes(int sx, int sy)
+{
+    if (sx < 0 || sx >= kNumShapes || sy < 0 || sy >= kNumShapes || shape_capacity(sx) < nx_ ||
+        shape_capacity(sy) < ny_)
+        throw Error(CEL_EINVAL, "line-solve block shape too small for the line length");
+    clearGraphs();
+    shape_x_ = sx;
+    shape_y_ = sy;
 }
 
 // Times each candidate on the current state.  The explicit kernel writes the next state's

@@ -119,6 +119,11 @@ public:
     void autotune(int repeats, cel_tuning *out) override;
     void getTuning(cel_tuning *out) const override;
     void setTuning(const cel_tuning &t) override;
+    // line-solve block shapes (indices into kShapeC / kShapeW) as a batch chooses them
+    int lineLength() const { return nx_ > ny_ ? nx_ : ny_; }     // the longer line direction
+    int shapeX() const { return shape_x_; }
+    int shapeY() const { return shape_y_; }
+    void setLineShapes(int sx, int sy);
     void sample(const int *xy, int n, float *out3) override;
     int timestepCount() const override { return timestep_count_; }
     void setVisual(bool on, float threshold, float decay) override;
