@@ -753,12 +753,21 @@ __device__ __forceinline__ Eq reduce_eq(const Eq &L, const Eq &H, const Eq &R)
 
 template <int C, int W>
 struct LineSmem {
-    // one unknown per thread above the chunk levels, as four planes: level j touches every
-    // 2^j-th thread, which a 16-byte record per thread would map onto a quarter of the banks
-    real ta[64 * W], tb[64 * W], tc[64 * W], td[64 * W];
+    // one unknown per thread above the chunk levels, as four planes.  Level j works on slots
+    // 2^j apart, dealt to consecutive lanes; slot i is kept at i + i/32, so those lanes fall
+    // on distinct banks for every j (a plain index would put up to 32 of them on one bank)
+    static constexpr int kPad = 64 * W + 2 * W;
+    real ta[kPad], tb[kPad], tc[kPad], td[kPad];
     Eq xch[2][64 * W];                         // chunk-level neighbour exchange (double-buffered)
-    __device__ Eq top(int i) const { return Eq{ta[i], tb[i], tc[i], td[i]}; }
-    __device__ void set_top(int i, const Eq &e) { ta[i] = e.a; tb[i] = e.b; tc[i] = e.c; td[i] = e.d; }
+    __device__ static int at(int i) { return i + (i >> 5); }
+    __device__ Eq top(int i) const { const int k = at(i); return Eq{ta[k], tb[k], tc[k], td[k]}; }
+    __device__ void set_top(int i, const Eq &e)
+    {
+        const int k = at(i);
+        ta[k] = e.a; tb[k] = e.b; tc[k] = e.c; td[k] = e.d;
+    }
+    __device__ real x(int i) const { return td[at(i)]; }
+    __device__ void set_x(int i, real v) { td[at(i)] = v; }
 };
 
 // Loads C consecutive elements starting at a 16-byte aligned element (C a multiple of
@@ -893,15 +902,15 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
         const int s = 1 << j;
         const int i = ((2 * t + 1) << j) - 1;
         if (i < NT && i * C + C - 1 < n) {
-            const real xl = i - s >= 0 ? sm.td[i - s] : 0.f;
-            const real xr = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.td[i + s] : 0.f;
+            const real xl = i - s >= 0 ? sm.x(i - s) : 0.f;
+            const real xr = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.x(i + s) : 0.f;
             const Eq e = sm.top(i);
-            sm.td[i] = (e.d - e.a * xl - e.c * xr) / e.b;
+            sm.set_x(i, (e.d - e.a * xl - e.c * xr) / e.b);
         }
         __syncthreads();
     }
-    q[C - 1].d = sm.td[t];
-    const real prev_last = t > 0 ? sm.td[t - 1] : 0.f;
+    q[C - 1].d = sm.x(t);
+    const real prev_last = t > 0 ? sm.x(t - 1) : 0.f;
     // ---- back-substitution inside the chunk: levels K0-1 .. 0
 #pragma unroll
     for (int k = K0 - 1; k >= 0; --k) {
