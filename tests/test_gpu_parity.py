@@ -151,12 +151,13 @@ def test_quadratic_friction_non_boussinesq():
     check(sc, a, b, exact=True)
 
 
-def test_plain_beach_1000_steps():
-    """256x256 beach with sine forcing, walls and an east sponge, 1000 steps."""
+def test_plain_beach_2000_steps():
+    """The plain 256x256 beach case for its full 2000 steps: sine forcing, walls and an east
+    sponge."""
     sc = S.plain_beach()
-    a, b = run_pair(sc, 1000)
+    a, b = run_pair(sc, 2000)
     err = check(sc, a, b)
-    print("plain_beach 1000 steps relative errors", err)
+    print("plain_beach 2000 steps relative errors", err)
 
 
 def test_long_lines_register_chunks():
