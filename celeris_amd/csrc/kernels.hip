@@ -405,7 +405,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     real nx_BX = ldrow(p.BX, ys), nx_BA = ldrow(p.BA, ys + 2);
     CellIn nx_c = load_cell<SCHEME, COMP>(p, (unsigned)ys * (unsigned)P + (unsigned)xc);
 
-    for (int y = ys; y < ye; ++y) {
+    auto row = [&](const int y) __attribute__((always_inline)) {
         // ---- rows y+2 (w, hu, hv, BY) and y (BX, BA, per-cell inputs), prefetched
         w_[4] = nx_w;
         hu_[3] = nx_hu;
@@ -631,7 +631,14 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         BY_[0] = BY_[1]; BY_[1] = BY_[2]; BY_[2] = BY_[3];
         Ncur = N1;
         yf_s0 = yf0; yf_s1 = yf1; yf_s2 = yf2;
+    };
+    // two rows per trip: the window shifts between them become register renames
+    int y = ys;
+    for (; y + 1 < ye; y += 2) {
+        row(y);
+        row(y + 1);
     }
+    if (y < ye) row(y);
     // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
     if (out_lane) {
         real *dst = p.DyT + (size_t)x * PT + (ys - 2);
