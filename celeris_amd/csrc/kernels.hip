@@ -632,13 +632,15 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         Ncur = N1;
         yf_s0 = yf0; yf_s1 = yf1; yf_s2 = yf2;
     };
-    // two rows per trip: the window shifts between them become register renames
+    // fast variant: two rows per trip, so that the window shifts between them become register
+    // renames (the plain variant's longer arithmetic would need more than 128 VGPRs for it)
     int y = ys;
-    for (; y + 1 < ye; y += 2) {
-        row(y);
-        row(y + 1);
-    }
-    if (y < ye) row(y);
+    if (FAST)
+        for (; y + 1 < ye; y += 2) {
+            row(y);
+            row(y + 1);
+        }
+    for (; y < ye; ++y) row(y);
     // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
     if (out_lane) {
         real *dst = p.DyT + (size_t)x * PT + (ys - 2);
