@@ -68,6 +68,14 @@ def lib():
             getattr(L, f"orc_{p}_shift_terrain").argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]
             getattr(L, f"orc_{p}_edit_water").argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 5 + [ctypes.c_int]
             getattr(L, f"orc_{p}_get_bottom").argtypes = [ctypes.c_void_p, R]
+            V = ctypes.c_float if p == "f32" else ctypes.c_double
+            getattr(L, f"orc_{p}_ab3_increment").restype = V
+            getattr(L, f"orc_{p}_ab3_increment").argtypes = [V] * 4
+            getattr(L, f"orc_{p}_am4_increment").restype = V
+            getattr(L, f"orc_{p}_am4_increment").argtypes = [V] * 5
+            getattr(L, f"orc_{p}_second_order_derivatives").argtypes = [V] * 5 + [R]
+            getattr(L, f"orc_{p}_time_series").argtypes = [_F, ctypes.c_int, ctypes.POINTER(ctypes.c_int), V, R]
+            getattr(L, f"orc_{p}_irregular_strip").argtypes = [ctypes.c_void_p, ctypes.c_int, R]
         L.orc_bottom_from_corners.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
@@ -206,6 +214,14 @@ class Oracle:
     def add_solitary_wave(self, H, theta_deg, x0, y0, refresh=False):
         getattr(lib(), f"orc_{self.p}_add_solitary_wave")(self.h, H, theta_deg, x0, y0, int(refresh))
 
+    def irregular_strip(self, side):
+        """(w, hu, hv) planes one irregular-wave side writes at the current time (zero
+        outside its strip)."""
+        dt = np.float32 if self.p == "f32" else np.float64
+        out = np.zeros((3, self.ny + 4, self.nx + 4), dt)
+        getattr(lib(), f"orc_{self.p}_irregular_strip")(self.h, side, fptr(out) if self.p == "f32" else dptr(out))
+        return out
+
     def coefficients(self):
         dt = np.float32 if self.p == "f32" else np.float64
         ax = np.zeros((3, self.ny + 4, self.nx + 4), dt)
@@ -238,3 +254,36 @@ def set_threads(n):
 
 def get_threads():
     return int(lib().orc_get_threads())
+
+
+def ab3_increment(dt, f0, fm1, fm2, precision="f64"):
+    return getattr(lib(), f"orc_{precision}_ab3_increment")(dt, f0, fm1, fm2)
+
+
+def am4_increment(dt, f1, f0, fm1, fm2, precision="f64"):
+    return getattr(lib(), f"orc_{precision}_am4_increment")(dt, f1, f0, fm1, fm2)
+
+
+def second_order_derivatives(y_n, y_n1, y_n2, dt_n1, dt_n2, precision="f64"):
+    dt = np.float32 if precision == "f32" else np.float64
+    out = np.zeros(3, dt)
+    getattr(lib(), f"orc_{precision}_second_order_derivatives")(y_n, y_n1, y_n2, dt_n1, dt_n2,
+                                                              fptr(out) if precision == "f32" else dptr(out))
+    return out
+
+
+class TimeSeries:
+    """The reference's uniform-time-series interpolation over rows (t, eta, hu, hv), with its
+    forward-only cursor (starts at row 1)."""
+
+    def __init__(self, rows, precision="f64"):
+        self.rows = np.ascontiguousarray(rows, np.float32)
+        self.p = precision
+        self.iter = ctypes.c_int(1)
+
+    def __call__(self, t):
+        dt = np.float32 if self.p == "f32" else np.float64
+        out = np.zeros(3, dt)
+        getattr(lib(), f"orc_{self.p}_time_series")(fptr(self.rows), self.rows.shape[0], ctypes.byref(self.iter), t,
+                                                    fptr(out) if self.p == "f32" else dptr(out))
+        return out

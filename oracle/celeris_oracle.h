@@ -150,6 +150,21 @@ float orc_f32_numerical_flux(float aplus, float aminus, float Fplus, float Fminu
 float orc_f32_sponge(float L, float x);
 float orc_f32_wavenumber(float omega, float d, float g);
 void orc_f32_ab_coef(float dt, float dt1, float dt2, float *coef3);
+/* Time integration (compute.hlsl:569-605, 755-798): the AB3 and AM4 increments at
+ * constant dt and the three second-order derivative estimates at unequal steps. */
+float orc_f32_ab3_increment(float dt, float f0, float fm1, float fm2);
+double orc_f64_ab3_increment(double dt, double f0, double fm1, double fm2);
+float orc_f32_am4_increment(float dt, float f1, float f0, float fm1, float fm2);
+double orc_f64_am4_increment(double dt, double f1, double f0, double fm1, double fm2);
+void orc_f32_second_order_derivatives(float y_n, float y_n1, float y_n2, float dt_n1, float dt_n2, float *der3);
+void orc_f64_second_order_derivatives(double y_n, double y_n1, double y_n2, double dt_n1, double dt_n2, double *der3);
+/* Uniform time series (timeSeriesHelper, engine.cpp:1887-1904) over n rows (t, eta, hu,
+ * hv) with the caller's cursor. */
+void orc_f32_time_series(const float *rows4, int n, int *iter, float t, float *out3);
+void orc_f64_time_series(const float *rows4, int n, int *iter, double t, double *out3);
+/* The (w, hu, hv) planes one irregular-wave side writes at the simulation's time. */
+void orc_f32_irregular_strip(orc_sim_f32 *s, int side, float *out3);
+void orc_f64_irregular_strip(orc_sim_f64 *s, int side, double *out3);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,8 @@ import ctypes
 import numpy as np
 import pytest
 
-from celeris_amd import SINEWAVE, SOLID, SPONGE, Boundary, bottom_from_corners, initial_state, settings
+from celeris_amd import (IRREGULARWAVES, SINEWAVE, SOLID, SPONGE, Boundary, bottom_from_corners, initial_state,
+                         settings)
 from celeris_amd import scenarios as S
 from oracle import binding as O
 
@@ -103,6 +104,153 @@ def test_adams_bashforth_coefficients():
     step = dt / 6.0 * (c[0] * f_n + c[1] * f_1 + c[2] * f_2)
     exact = ((t0 + dt) ** 3 - t0 ** 3) / 3.0
     assert step == pytest.approx(exact, rel=1e-5)
+
+
+def _poly(c, t):
+    return sum(ci * t ** i for i, ci in enumerate(c))
+
+
+def _poly_int(c, a, b):
+    return sum(ci * (b ** (i + 1) - a ** (i + 1)) / (i + 1) for i, ci in enumerate(c))
+
+
+def test_ab3_increment_exact_for_quadratic_rates():
+    # dt/12 (23 f_n - 16 f_{n-1} + 5 f_{n-2}) integrates y' = p(t) exactly for deg p <= 2
+    for c in ([1.0], [0.3, -2.0], [0.5, 1.5, -3.0]):
+        for T, h in ((1.0, 0.1), (2.5, 0.37)):
+            f = [_poly(c, T - k * h) for k in range(3)]
+            got = O.ab3_increment(h, *f)
+            assert got == pytest.approx(_poly_int(c, T, T + h), rel=1e-12, abs=1e-15)
+    # a cubic rate is not integrated exactly (third order): the check has teeth
+    c = [0.0, 0.0, 0.0, 1.0]
+    f = [_poly(c, 1.0 - k * 0.5) for k in range(3)]
+    assert abs(O.ab3_increment(0.5, *f) - _poly_int(c, 1.0, 1.5)) > 1e-3
+    assert O.ab3_increment(0.1, 1.0, 1.0, 1.0, "f32") == pytest.approx(0.1, rel=1e-6)
+
+
+def test_am4_increment_exact_for_cubic_rates():
+    # dt/24 (9 f_{n+1} + 19 f_n - 5 f_{n-1} + f_{n-2}) integrates y' = p(t) exactly for deg p <= 3
+    for c in ([1.0], [0.2, -1.0, 0.7], [0.5, 1.5, -3.0, 2.0]):
+        for T, h in ((1.0, 0.1), (3.0, 0.45)):
+            f1 = _poly(c, T + h)
+            f0, fm1, fm2 = (_poly(c, T - k * h) for k in range(3))
+            got = O.am4_increment(h, f1, f0, fm1, fm2)
+            assert got == pytest.approx(_poly_int(c, T, T + h), rel=1e-12, abs=1e-15)
+    # swapping the weights of f_{n+1} and f_n breaks exactness on y' = t^3
+    c = [0.0, 0.0, 0.0, 1.0]
+    T, h = 1.0, 0.5
+    f1 = _poly(c, T + h)
+    f0, fm1, fm2 = (_poly(c, T - k * h) for k in range(3))
+    assert abs(O.am4_increment(h, f0, f1, fm1, fm2) - _poly_int(c, T, T + h)) > 1e-2
+    # a quartic rate is not integrated exactly (fourth order)
+    c = [0.0, 0.0, 0.0, 0.0, 1.0]
+    f1 = _poly(c, T + h)
+    f0, fm1, fm2 = (_poly(c, T - k * h) for k in range(3))
+    assert abs(O.am4_increment(h, f1, f0, fm1, fm2) - _poly_int(c, T, T + h)) > 1e-4
+    assert O.am4_increment(0.2, 1.0, 1.0, 1.0, 1.0, "f32") == pytest.approx(0.2, rel=1e-6)
+
+
+@pytest.mark.parametrize("h1,h2", [(0.1, 0.1), (0.2, 0.45), (0.7, 0.05)])
+def test_second_order_derivatives_exact_for_quadratics(h1, h2):
+    # y = c0 + c1 t + c2 t^2 sampled at t_n, t_n - h1, t_n - h1 - h2: the three one-sided /
+    # central second-order estimates equal y' at those three times exactly
+    T = 1.3
+    ts = (T, T - h1, T - h1 - h2)
+    for c in ([2.0, -1.0, 0.5], [0.0, 0.0, 1.0], [1.0, 3.0, 0.0]):
+        y = [_poly(c, t) for t in ts]
+        dy = [c[1] + 2 * c[2] * t for t in ts]
+        np.testing.assert_allclose(O.second_order_derivatives(*y, h1, h2), dy, rtol=1e-11, atol=1e-12)
+        np.testing.assert_allclose(O.second_order_derivatives(*y, h1, h2, precision="f32"), dy, rtol=2e-4,
+                                   atol=2e-4)
+    # a cubic is not differentiated exactly
+    y = [t ** 3 for t in ts]
+    assert abs(O.second_order_derivatives(*y, h1, h2)[0] - 3 * T ** 2) > 1e-4
+
+
+def test_time_series_interpolation_cursor_and_end():
+    # rows (t, eta, hu, hv) of three straight pieces with different slopes
+    rows = np.array([[0.0, 0.0, 1.0, -1.0],
+                     [0.5, 0.25, 0.0, 0.0],
+                     [1.5, -0.75, 2.0, 1.0],
+                     [2.0, 0.25, 2.0, 3.0]], np.float32)
+    ts = O.TimeSeries(rows)
+
+    def piece(t, i):      # linear through rows i-1 and i
+        a, b = rows[i - 1].astype(np.float64), rows[i].astype(np.float64)
+        m = (t - a[0]) / (b[0] - a[0])
+        return a[1:] + (b[1:] - a[1:]) * m
+
+    np.testing.assert_allclose(ts(0.25), piece(0.25, 1), rtol=1e-12)
+    np.testing.assert_allclose(ts(0.5), rows[1, 1:], rtol=1e-12)          # a row time: that row
+    np.testing.assert_allclose(ts(1.0), piece(1.0, 2), rtol=1e-12)
+    np.testing.assert_allclose(ts(1.5), rows[2, 1:], rtol=1e-12, atol=1e-12)
+    assert ts.iter.value == 2
+    # the cursor only moves forward: an earlier time extrapolates the current piece
+    np.testing.assert_allclose(ts(0.25), piece(0.25, 2), rtol=1e-12)
+    np.testing.assert_allclose(ts(1.75), piece(1.75, 3), rtol=1e-12)
+    # past the last row the series is zero, and stays zero
+    np.testing.assert_array_equal(ts(2.5), [0.0, 0.0, 0.0])
+    np.testing.assert_array_equal(ts(1.0), [0.0, 0.0, 0.0])
+    f32 = O.TimeSeries(rows, "f32")
+    np.testing.assert_allclose(f32(1.0), piece(1.0, 2), rtol=1e-6)
+
+
+def _irregular_case(n_waves, sl, theta, nx=40, ny=20, seed=0):
+    rng = np.random.default_rng(seed)
+    dx = 0.5
+    W, L = dx * (nx - 1), dx * (ny - 1)
+    X, Y = S.corner_coords(nx, ny, W, L)
+    z = -1.0 + 0.0 * X
+    z[-6:, :] = 1.0                        # the northmost rows are dry land
+    waves = np.stack([rng.uniform(0.01, 0.05, n_waves), rng.uniform(1.0, 4.0, n_waves),
+                      np.full(n_waves, theta), rng.uniform(0, 2 * np.pi, n_waves)], axis=1).astype(np.float32)
+    s = settings(nx, ny, W, L, 0.01, still_water_elevation=sl)
+    bnd = [Boundary(IRREGULARWAVES, 2, sl, waves=waves), Boundary(SOLID, 2), Boundary(SOLID, 2), Boundary(SOLID, 2)]
+    bottom = bottom_from_corners(nx, ny, z.astype(np.float32))
+    return s, bnd, bottom, initial_state(s, bottom, []), waves
+
+
+@pytest.mark.parametrize("n_waves", [1, 3, 37])
+@pytest.mark.parametrize("sl,theta", [(0.0, 0.0), (0.2, 0.0), (0.1, 0.4)])
+def test_irregular_strip_is_the_truncated_component_sum(n_waves, sl, theta):
+    # nx = 40: L = 6 reduction levels, so the tree sums the first 2^5 = 32 texels of each line.
+    # Each texel carries eta_i + sl: the line value is sum_{i < min(n, 32)} eta_i + 32 sl, with
+    # eta_i = a_i sin(w_i t - k_i x cos(th_i) - k_i y sin(th_i) + phi_i) on wet cells (depth
+    # sl - BA > 1e-4) and 0 on dry ones; the two ghost cells at each end of a line are 0.
+    s, bnd, bottom, state, waves = _irregular_case(n_waves, sl, theta)
+    o = O.Oracle(s, bnd, bottom, state, "f64")
+    o.step(3)
+    t = o.time
+    strip = o.irregular_strip(0)
+    nx, ny = s.nx, s.ny
+    Xn, Yn = nx + 4, ny + 4
+    dx, dy = s.width / (nx - 1), s.length / (ny - 1)
+    leaves = 32
+    g = float(np.float32(s.gravity))       # the configuration's FP32 values, as the oracle reads them
+    sl = float(np.float32(sl))
+    a, T, th, ph = (waves[:min(n_waves, leaves), k].astype(np.float32).astype(np.float64) for k in range(4))
+    om = 2 * np.pi / T
+    for col in (0, 1):
+        for row in range(Yn):
+            w, hu, hv = strip[:, row, col]
+            if row < 2 or row >= Yn - 2:
+                assert w == hu == hv == 0.0
+                continue
+            d = max(0.0, sl - float(bottom[2, row, col]))
+            if d > 1e-4:
+                k = om ** 2 / (g * np.sqrt(np.tanh(om ** 2 * d / g)))     # Eckart, pinned above
+                eta = a * np.sin(om * t - np.cos(th) * (col * dx) * k - np.sin(th) * (row * dy) * k + ph)
+                c = om / k
+                exp = (eta.sum() + leaves * sl, (c * np.cos(th) * eta).sum(), (c * np.sin(th) * eta).sum())
+            else:
+                exp = (leaves * sl, 0.0, 0.0)
+            np.testing.assert_allclose((w, hu, hv), exp, rtol=1e-9, atol=1e-12)
+    # columns outside the strip are untouched
+    assert not strip[:, :, 2:].any()
+    # the FP32 oracle agrees to single precision
+    o32 = O.Oracle(s, bnd, bottom, state, "f32")
+    o32.step(3)
+    np.testing.assert_allclose(o32.irregular_strip(0), strip, rtol=0, atol=2e-5 * max(1.0, leaves * sl))
 
 
 # ---------------------------------------------------------------- tridiagonal
