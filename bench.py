@@ -132,6 +132,28 @@ def serving_leg(n, rates, seed, block):
     return out
 
 
+def device_bandwidth(nbytes=1 << 30, launches=20):
+    """The library's HBM streaming probes (read, write, copy) on this GPU: GB/s of bytes moved."""
+    from celeris_amd import probe_bandwidth
+    return {k: round(probe_bandwidth(k, nbytes, launches)[1], 1) for k in ("read", "write", "copy")}
+
+
+def bandwidth_line():
+    from celeris_amd import probe_bandwidth
+    rows = {}
+    for nbytes in (256 << 20, 1 << 30, 4 << 30):
+        for kind in ("read", "write", "copy"):
+            ms, gbs = probe_bandwidth(kind, nbytes, 20)
+            rows[f"{kind} {nbytes >> 20} MiB"] = {"ms": ms, "gb_per_s": gbs}
+            print(f"bench: probe {kind} {nbytes >> 20} MiB: {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    best = max(v["gb_per_s"] for k, v in rows.items() if not k.endswith(" 256 MiB"))
+    return {"metric": "HBM streaming rate (library probes)", "value": best, "unit": "GB/s", "n_gpus": 1,
+            "steps": 20, "warmup": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "read / write / copy of 256 MiB, 1 GiB and 4 GiB device "
+                                                     "buffers, 16 KiB per workgroup", "runs": rows,
+                                          "peak_gb_per_s": HBM_PEAK_GBS}}
+
+
 def ensemble_line(args):
     from celeris_amd import Batch, ShallowWaterEngine
     from celeris_amd import scenarios as S
@@ -241,29 +263,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
-    ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu", "bandwidth"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
     ap.add_argument("--serving-rates", type=str, default="5,50", help="Poisson arrival rates (1/s), comma-separated")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy a 1 GiB device buffer 4 times (reference for the HBM byte counters)")
+    ap.add_argument("--probe", choices=["read", "write", "copy"], default=None,
+                    help="only run the library's HBM streaming probe of this kind on a 1 GiB buffer (4 launches: the "
+                         "known-size reference for the HBM byte counters)")
     args = ap.parse_args()
-    if args.copy_calibration:
-        # 1 GiB source and destination: four times the 256 MB last-level cache, so every
-        # copy reads and writes HBM
-        # (copy: 1 GiB written per launch; sum: 1 GiB read per launch)
-        import torch
-        n = 256 * 1024 * 1024
-        a = torch.ones(n, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(4):
-            b.copy_(a)
-        s = 0.0
-        for _ in range(4):
-            s += float(b.sum())
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": 4 * n, "sum_bytes_per_launch": 4 * n, "launches": 4, "check": s}))
+    if args.probe:
+        from celeris_amd import probe_bandwidth
+        ms, gbs = probe_bandwidth(args.probe, 1 << 30, 4)
+        print(json.dumps({"probe": args.probe, "bytes_per_launch": 1 << 30, "launches": 5, "ms_per_launch": ms,
+                          "gb_per_s": gbs}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -276,6 +289,9 @@ def main():
         if "--nx" not in sys.argv:
             args.nx, args.ny = 4096, 1024
         print(json.dumps(precision_line(args)), flush=True)
+        return
+    if args.workload == "bandwidth":
+        print(json.dumps(bandwidth_line()), flush=True)
         return
     if args.workload == "cpu":
         rows = cpu_baselines()

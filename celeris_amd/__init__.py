@@ -124,6 +124,7 @@ SIGNATURES = {
     "cel_engine_stats": ([ctypes.c_void_p, ctypes.POINTER(CelStats)], ctypes.c_int),
     "cel_engine_reset_timestep": ([ctypes.c_void_p, ctypes.POINTER(CelAdaptive), ctypes.POINTER(CelStats)],
                                   ctypes.c_int),
+    "cel_probe_bandwidth": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _F], ctypes.c_int),
 }
 
 _lib = None
@@ -213,6 +214,18 @@ def hot_start_state(s, bottom, hot, solitons=()):
     _check(load_library().cel_hot_start_state(ctypes.byref(s), _fptr(bottom), _fptr(hot), arr, len(solitons),
                                               _fptr(out)))
     return out
+
+
+PROBE_KINDS = {"read": 0, "write": 1, "copy": 2}
+
+
+def probe_bandwidth(kind, nbytes=1 << 30, launches=20):
+    """Average launch time (ms) and rate (GB/s of bytes moved: read or written bytes, twice the
+    size for a copy) of the library's HBM streaming probe over a device buffer of nbytes."""
+    ms = ctypes.c_float()
+    _check(load_library().cel_probe_bandwidth(PROBE_KINDS[kind], nbytes, launches, ctypes.byref(ms)))
+    moved = nbytes * (2 if kind == "copy" else 1)
+    return ms.value, moved / (ms.value * 1e-3) / 1e9
 
 
 class ShallowWaterEngine:

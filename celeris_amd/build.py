@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libceleris_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-# (source, object, extra defines)
+# (source, object, extra defines[, extra flags])
 UNITS = [
     ("kernels.hip", "kernels.o", []),
     ("kernels.hip", "kernels64.o", ["CEL_FP64"]),
@@ -24,6 +24,7 @@ UNITS = [
     ("batch.cpp", "batch.o", []),
     ("terrain.cpp", "terrain.o", []),
     ("capi.cpp", "capi.o", []),
+    ("probe.hip", "probe.o", []),
 ]
 SOURCES = sorted({u[0] for u in UNITS})
 
@@ -51,9 +52,10 @@ def build(verbose=False, out=LIB, defines=(), jobs=None):
     tag = "" if not defines else "_" + "_".join(d.split("=")[0].lower() for d in defines)
 
     def compile_unit(unit):
-        src, obj, extra = unit
+        src, obj, extra = unit[:3]
+        flags = unit[3] if len(unit) > 3 else []
         obj_path = os.path.join(BUILD_DIR, obj.replace(".o", tag + ".o"))
-        cmd = [_hipcc()] + COMPILE_FLAGS + [f"-D{d}" for d in list(defines) + extra]
+        cmd = [_hipcc()] + COMPILE_FLAGS + flags + [f"-D{d}" for d in list(defines) + extra]
         cmd += ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj_path]
         r = subprocess.run(cmd, capture_output=True, text=True)
         return obj_path, r.returncode, _filter(r.stdout + r.stderr)

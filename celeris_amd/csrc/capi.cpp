@@ -14,6 +14,7 @@ namespace cel {
 void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
 void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3,
                    const float *base3);
+float probe_bandwidth(int kind, size_t bytes, int launches);
 }
 
 struct cel_engine {
@@ -371,6 +372,14 @@ int cel_engine_reset_timestep(cel_engine *e, const cel_adaptive *a, cel_stats *o
         if (!(a->ema_alpha >= 0.f && a->ema_alpha <= 1.f)) throw cel::Error(CEL_EINVAL, "ema_alpha must lie in [0, 1]");
         if (!(a->min_timestep_ratio > 0.f)) throw cel::Error(CEL_EINVAL, "min_timestep_ratio must be positive");
         eng(e).resetTimestepAdaptive(*a, out);
+    });
+}
+
+int cel_probe_bandwidth(int kind, size_t bytes, int launches, float *ms)
+{
+    return guarded([&] {
+        const float t = cel::probe_bandwidth(kind, bytes, launches);
+        if (ms) *ms = t;
     });
 }
 

@@ -63,6 +63,7 @@ typedef enum {
  * cel_engine_get_state_f64.  Not available in batches.  No counterpart in the reference. */
 typedef enum { CEL_STATE_FP32 = 0, CEL_STATE_COMPENSATED = 1, CEL_STATE_FP64 = 2 } cel_state_precision;
 
+
 /* Maximum irregular-wave components per side: MAX_NUM_OF_IRREGULAR_WAVES,
  * engine.hpp:216 / compute.hlsl:111-114. */
 #define CEL_MAX_IRREGULAR_WAVES 1000
@@ -340,6 +341,13 @@ int cel_engine_set_tuning(cel_engine *e, const cel_tuning *t);
 /* Level-0 tridiagonal coefficients as the engine uses them (engine.cpp:2638-2694,
  * 2781-2826): three planes A, B, C for x and for y, computed on the device. */
 int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3);
+
+/* HBM streaming probe: kind 0 reads, 1 writes, 2 copies `bytes` (a multiple of 16 KiB, at
+ * most 16 GiB) in one launch of 16 KiB per workgroup; `launches` timed launches after one
+ * warm-up, *ms = average launch time.  Gives the device's achievable streaming rates and a
+ * known-size launch for calibrating the HBM byte counters.  Allocates its own buffers.  No
+ * counterpart in the reference. */
+int cel_probe_bandwidth(int kind, size_t bytes, int launches, float *ms);
 
 #ifdef __cplusplus
 }

@@ -13,40 +13,49 @@ from tests.parity import eta, rel_err, wet_mask
 pytestmark = pytest.mark.gpu
 
 
-def test_headline_2048_matches_oracle():
-    """2048^2, 100 irregular-wave components on the west side, 20-cell east sponge: Euler
-    bootstrap and AB3 steps within 1e-5 of the FP32 oracle (device sinf), same wet/dry mask."""
+def test_headline_2048_1000_steps_matches_oracle():
+    """2048^2, 100 irregular-wave components on the west side, 20-cell east sponge, after 1000
+    steps (Euler bootstrap, then AB3): within 1e-5 of the FP32 oracle (device sinf against the
+    C library's) with the same wet/dry mask."""
     sc = S.headline()
-    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
-    e.timestep(24)
-    o.step(24)
-    a, b = e.get_state(), o.state()
+    o.step(1000)
+    b = o.state()
+    del o
+    assert np.abs(b[1]).max() > 0                    # the waves have entered
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(1000)
+    a = e.get_state()
     e.close()
     assert np.isfinite(a).all()
     err = rel_err(eta(a), eta(b))
+    print("headline 2048^2, 1000 steps, relative errors", err)
     assert max(err.values()) < 1e-5, err
     assert (wet_mask(a, sc.bottom) == wet_mask(b, sc.bottom)).all()
-    assert np.abs(b[1]).max() > 0                    # the waves have entered
 
 
 def test_precision_channel_4096x1024_matches_oracle():
     """The precision case at its full size: FP32 engine == FP32 oracle bit for bit (no
-    trigonometric forcing); the FP64 engine equals the FP64 oracle to round-off."""
+    trigonometric forcing) after 200 steps; the FP64 engine equals the FP64 oracle to
+    round-off after 1000 steps, with the solitary wave moved about 20 cells."""
     sc = S.solitary_channel()
     e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
-    e.timestep(16)
-    o.step(16)
+    e.timestep(200)
+    o.step(200)
     assert np.array_equal(e.get_state(), o.state())
     e.close()
+    del o
     sc.s.state_precision = STATE_FP64
     e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
     o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f64")
-    e.timestep(8)
-    o.step(8)
-    np.testing.assert_allclose(e.get_state_f64(), o.state(), rtol=0, atol=1e-12)
+    e.timestep(1000)
+    o.step(1000)
+    b = o.state()
+    np.testing.assert_allclose(e.get_state_f64(), b, rtol=0, atol=1e-11)
     e.close()
+    # the crest has moved: the surface differs from the initial state by a good part of H
+    assert np.abs(b[0] - sc.state[0].astype(np.float64)).max() > 0.05
 
 
 def bowl(nx=2048, dx=1.0, seed=1):
