@@ -125,6 +125,10 @@ SIGNATURES = {
     "cel_engine_reset_timestep": ([ctypes.c_void_p, ctypes.POINTER(CelAdaptive), ctypes.POINTER(CelStats)],
                                   ctypes.c_int),
     "cel_probe_bandwidth": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _F], ctypes.c_int),
+    "cel_engine_check_numeric": ([ctypes.c_void_p], ctypes.c_int),
+    "cel_batch_check_numeric": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_batch_set_state": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
 }
 
 _lib = None
@@ -355,6 +359,15 @@ class ShallowWaterEngine:
         _check(load_library().cel_engine_timed_steps(self._h, n, ctypes.byref(ms)))
         return ms.value
 
+    def numeric_ok(self):
+        """False once a step has produced a non-finite value (the device sentinel); waits for
+        the engine's work."""
+        rc = load_library().cel_engine_check_numeric(self._h)
+        if rc == 4:
+            return False
+        _check(rc)
+        return True
+
     def set_profiling(self, on):
         _check(load_library().cel_engine_set_profiling(self._h, int(bool(on))))
 
@@ -504,6 +517,22 @@ class Batch:
         out = np.zeros((3, ny + 4, nx + 4), np.float32)
         _check(load_library().cel_batch_get_state(self._h, sid, _fptr(out)))
         return out
+
+    def set_state(self, sid, state):
+        """Hot start of one member (clears its numeric sentinel)."""
+        nx, ny = self._dim(sid)
+        state = np.ascontiguousarray(state, dtype=np.float32)
+        assert state.shape == (3, ny + 4, nx + 4)
+        _check(load_library().cel_batch_set_state(self._h, sid, _fptr(state)))
+
+    def failed(self):
+        """Ids of the members whose steps produced a non-finite value (device sentinel); waits
+        for the batch's work."""
+        n = ctypes.c_int()
+        cap = max(1, len(self._dims))
+        ids = (ctypes.c_int * cap)()
+        _check(load_library().cel_batch_check_numeric(self._h, ids, cap, ctypes.byref(n)))
+        return [ids[i] for i in range(min(n.value, cap))]
 
     def get_state_f64(self, sid):
         nx, ny = self._dim(sid)

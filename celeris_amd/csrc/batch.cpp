@@ -21,6 +21,11 @@ Batch::Batch()
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &e : blob_ev_) check(hipEventCreate(&e), "hipEventCreate");
     for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+    void *mem = nullptr;
+    check(hipMalloc(&mem, kMaxMembers * sizeof(int)), "hipMalloc(batch status)");
+    d_status_ = static_cast<int *>(mem);
+    check(hipMemsetAsync(d_status_, 0, kMaxMembers * sizeof(int), stream_), "clear batch status");
+    for (int i = kMaxMembers - 1; i >= 0; --i) free_slots_.push_back(i);
 }
 
 Batch::~Batch()
@@ -31,6 +36,7 @@ Batch::~Batch()
     if (d_blob_) (void)hipFree(d_blob_);
     for (auto &h : h_blob_) if (h) (void)hipHostFree(h);
     if (d_starts_) (void)hipFree(d_starts_);
+    if (d_status_) (void)hipFree(d_status_);
     for (auto &e : blob_ev_) if (e) (void)hipEventDestroy(e);
     for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -40,12 +46,37 @@ int Batch::add(const cel_settings &s, const cel_boundary bnd[4], const float *bo
 {
     if (s.correction_steps_num != 0) throw Error(CEL_EINVAL, "batched scenarios do not take corrector steps");
     if (s.state_precision == CEL_STATE_FP64) throw Error(CEL_EINVAL, "batched scenarios run in FP32 (plain or compensated)");
+    if (free_slots_.empty()) throw Error(CEL_ENOMEM, "a batch holds at most 4096 scenarios");
     sync();
     auto e = std::make_unique<ShallowWaterEngine>(s, bnd, bottom3, state3);
     const int id = next_id_++;
+    const int slot = free_slots_.back();
+    check(hipMemsetAsync(d_status_ + slot, 0, sizeof(int), stream_), "clear member status");
+    free_slots_.pop_back();
+    slot_[id] = slot;
     scen_[id] = std::move(e);
     comp_valid_ = false;
     return id;
+}
+
+void Batch::setState(int id, const float *state3)
+{
+    ShallowWaterEngine &e = scenario(id);
+    e.setState(state3);
+    check(hipMemsetAsync(d_status_ + slot_.at(id), 0, sizeof(int), stream_), "clear member status");
+    sync();
+}
+
+std::vector<int> Batch::failed()
+{
+    std::vector<int> flags(kMaxMembers);
+    check(hipMemcpyAsync(flags.data(), d_status_, kMaxMembers * sizeof(int), hipMemcpyDeviceToHost, stream_),
+          "read batch status");
+    sync();
+    std::vector<int> out;
+    for (const auto &kv : slot_)
+        if (flags[kv.second] != 0) out.push_back(kv.first);
+    return out;
 }
 
 void Batch::remove(int id)
@@ -54,6 +85,8 @@ void Batch::remove(int id)
     if (it == scen_.end()) throw Error(CEL_EINVAL, "no such scenario");
     sync();
     scen_.erase(it);
+    free_slots_.push_back(slot_.at(id));
+    slot_.erase(id);
     paused_.erase(id);
     comp_valid_ = false;
 }
@@ -267,6 +300,7 @@ void Batch::fillTable(const std::vector<StepLaunch> &plans, const std::vector<Sh
         for (int i = 0; i < ns; ++i) {
             const StepLaunch &L = plans[(size_t)k * ns + i];
             ej[i].p = L.ep;
+            ej[i].p.status = d_status_ + slot_.at(comp_.ids[i]);
             ej[i].scheme = L.scheme;
             ej[i].comp = eng[i]->compensated();
             ej[i].fast = eng[i]->fastArith();

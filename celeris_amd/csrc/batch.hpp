@@ -42,6 +42,10 @@ public:
     void sync();
     float timedSteps(int nsteps);
     ShallowWaterEngine &scenario(int id);      // synchronises the batch stream first
+    void setState(int id, const float *state3);   // hot start of one member (clears its sentinel)
+    // members whose steps produced a non-finite value since they were added or last given a
+    // state (waits for the stream)
+    std::vector<int> failed();
     int count() const { return (int)scen_.size(); }
     size_t deviceBytes() const;
     void ids(int *out) const;
@@ -71,6 +75,11 @@ private:
 
     hipStream_t stream_ = nullptr;
     std::map<int, std::unique_ptr<ShallowWaterEngine>> scen_;
+    // numeric sentinel words, one slot per member (ExplicitParams::status of its jobs)
+    static constexpr int kMaxMembers = 4096;
+    int *d_status_ = nullptr;
+    std::map<int, int> slot_;                  // member id -> sentinel slot
+    std::vector<int> free_slots_;
     std::set<int> paused_;
     int next_id_ = 1;
     int graph_steps_ = 0;

@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points (include/celeris.h).  Exceptions never cross the ABI:
 // each call maps cel::Error / std::bad_alloc to a status and records the message.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -312,6 +313,32 @@ int cel_batch_device_bytes(cel_batch *b, size_t *bytes)
     return guarded([&] {
         if (!bytes) throw cel::Error(CEL_EINVAL, "null output");
         *bytes = bat(b).deviceBytes();
+    });
+}
+
+int cel_engine_check_numeric(cel_engine *e)
+{
+    return guarded([&] {
+        if (eng(e).numericStatus() != 0)
+            throw cel::Error(CEL_ENUMERIC, "a step produced a non-finite surface elevation or line right-hand side");
+    });
+}
+
+int cel_batch_check_numeric(cel_batch *b, int *ids, int max_ids, int *n_failed)
+{
+    return guarded([&] {
+        if (max_ids < 0 || (max_ids > 0 && !ids)) throw cel::Error(CEL_EINVAL, "bad id buffer");
+        const std::vector<int> f = bat(b).failed();
+        for (int i = 0; i < (int)f.size() && i < max_ids; ++i) ids[i] = f[i];
+        if (n_failed) *n_failed = (int)f.size();
+    });
+}
+
+int cel_batch_set_state(cel_batch *b, int id, const float *state3)
+{
+    return guarded([&] {
+        if (!state3) throw cel::Error(CEL_EINVAL, "null state");
+        bat(b).setState(id, state3);
     });
 }
 

@@ -143,6 +143,8 @@ void ShallowWaterEngine::allocate()
     check(hipMalloc(&mem, sbytes + 256), "hipMalloc(stats)");
     d_stats_ = static_cast<double *>(mem);
     d_flag_ = reinterpret_cast<int *>(static_cast<char *>(mem) + sbytes);   // after the partial sums
+    d_status_ = d_flag_ + 1;
+    check(hipMemsetAsync(d_status_, 0, sizeof(int), stream_), "clear status");
     table_bytes_ += sbytes + 256;
     check(hipHostMalloc(&mem, kStatsValues * sizeof(double), hipHostMallocDefault), "hipHostMalloc(stats)");
     h_stats_ = static_cast<double *>(mem);
@@ -264,8 +266,17 @@ void ShallowWaterEngine::setState(const float *state3)
         }
     }
     if (aux_on_) zeroAux();
+    check(hipMemsetAsync(d_status_, 0, sizeof(int), stream_), "clear status");
     check(hipStreamSynchronize(stream_), "upload sync");
     resetScheme();
+}
+
+int ShallowWaterEngine::numericStatus()
+{
+    int v = 0;
+    check(hipMemcpyAsync(&v, d_status_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read status");
+    check(hipStreamSynchronize(stream_), "status sync");
+    return v;
 }
 
 // Device planes (w, hu row-major, hv transposed) of one buffer set -> three host planes.
@@ -570,6 +581,7 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     p.Cw = Cw_; p.Chu = Chu_; p.ChvT = ChvT_;
     p.w_new = w_[n]; p.Dx = Dx_; p.DyT = DyT_;
     p.w_lo = w_lo_[c]; p.w_lo_new = w_lo_[n];
+    p.status = d_status_;
     // fillConstantBuffers, engine.cpp:4099-4136
     const real g = s_.gravity;
     p.two_theta = real(2) * (real)s_.theta;
@@ -983,24 +995,7 @@ void ShallowWaterEngine::runStats(double *out9)
 // resetTimestep's post-processing of the reductions, engine.cpp:1686-1696 (in FP64)
 void ShallowWaterEngine::deriveStats(const double *r, cel_stats *o) const
 {
-    std::memset(o, 0, sizeof(*o));
-    o->sum_h = r[0]; o->sum_bh_h2 = r[1]; o->sum_hu = r[2]; o->sum_hv = r[3]; o->sum_hu2v2 = r[4];
-    o->max_u2v2 = r[5]; o->max_h = r[6]; o->max_cfl_rate = r[7]; o->max_u2v2_h = r[8];
-    const double density = 1000.0;
-    const double area = (double)s_.width / (double)(nx_ - 1) * ((double)s_.length / (double)(ny_ - 1));
-    const double g = s_.gravity;
-    o->mass = r[0] * density * area;
-    o->x_momentum = r[2] * density * area;
-    o->y_momentum = r[3] * density * area;
-    o->kinetic_energy = 0.5 * density * area * r[4];
-    o->potential_energy = g * density * area * r[1];
-    o->total_energy = o->kinetic_energy + o->potential_energy;
-    o->max_speed = std::sqrt(r[5]);
-    o->max_depth = r[6];
-    o->max_froude = std::sqrt(r[8] / g);
-    o->timestep = (float)current_timestep_;
-    o->cfl_number = r[7] * (double)current_timestep_;
-    o->cells = 16 * (nx_ / 4) * (ny_ / 4);
+    cel::derive_stats(s_, (double)current_timestep_, r, o);
 }
 
 void ShallowWaterEngine::stats(cel_stats *out)

@@ -124,6 +124,10 @@ public:
     int shapeX() const { return shape_x_; }
     int shapeY() const { return shape_y_; }
     void setLineShapes(int sx, int sy);
+    // the numeric sentinel: nonzero once a step produced a non-finite value (waits for the
+    // stream); setState clears it
+    int numericStatus() override;
+    int *statusWord() const { return d_status_; }
     void sample(const int *xy, int n, float *out3) override;
     int timestepCount() const override { return timestep_count_; }
     void setVisual(bool on, float threshold, float decay) override;
@@ -216,6 +220,7 @@ private:
     // the explicit kernel's FAST variant (shortened exact division and square root): the
     // mesh and settings admit it (static_fields' check, fastSettings) and the caller allows it
     int *d_flag_ = nullptr;
+    int *d_status_ = nullptr;        // numeric sentinel of the explicit kernel (ExplicitParams::status)
     bool fast_mesh_ = false, fast_allowed_ = true;
     bool fastSettings() const;
 

@@ -6,12 +6,38 @@
 // read-back.
 #pragma once
 
+#include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
 #include "../../include/celeris.h"
 
 namespace cel {
+
+// resetTimestep's post-processing of GetStats' nine reductions (engine.cpp:1686-1696), in
+// FP64: density 1000 kg/m^3, cell area dx dy; dt is the next step's.
+inline void derive_stats(const cel_settings &s, double dt, const double *r, cel_stats *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->sum_h = r[0]; o->sum_bh_h2 = r[1]; o->sum_hu = r[2]; o->sum_hv = r[3]; o->sum_hu2v2 = r[4];
+    o->max_u2v2 = r[5]; o->max_h = r[6]; o->max_cfl_rate = r[7]; o->max_u2v2_h = r[8];
+    const double density = 1000.0;
+    const double area = (double)s.width / (double)(s.nx - 1) * ((double)s.length / (double)(s.ny - 1));
+    const double g = s.gravity;
+    o->mass = r[0] * density * area;
+    o->x_momentum = r[2] * density * area;
+    o->y_momentum = r[3] * density * area;
+    o->kinetic_energy = 0.5 * density * area * r[4];
+    o->potential_energy = g * density * area * r[1];
+    o->total_energy = o->kinetic_energy + o->potential_energy;
+    o->max_speed = std::sqrt(r[5]);
+    o->max_depth = r[6];
+    o->max_froude = std::sqrt(r[8] / g);
+    o->timestep = (float)dt;
+    o->cfl_number = r[7] * dt;
+    o->cells = 16 * (s.nx / 4) * (s.ny / 4);
+}
 
 struct Error : std::runtime_error {
     int status;
@@ -51,6 +77,7 @@ public:
     virtual void raiseTerrain(float world_x, float world_y, float radius, float displacement) = 0;
     virtual void shiftTerrain(float shift_up) = 0;
     virtual void editWater(float world_x, float world_y, float radius, float disp_A, float disp_B) = 0;
+    virtual int numericStatus() = 0;
 };
 
 }  // namespace cel

@@ -399,6 +399,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     real yf_s0, yf_s1, yf_s2;   // yflux at row y-1
     flux_y<FAST>(g, half_g, Nprev, Scur, yf_s0, yf_s1, yf_s2);
 
+    // a lane's new values were not all finite (numeric sentinel)
+    bool bad = false;
+
     // software pipelining: the loads a row needs are issued one row ahead
     const int ylast = p.ny + 3;
     real nx_w = ldrow(p.w, ys + 2), nx_hu = ldrow(p.hu, ys + 2), nx_BY = ldrow(p.BY, ys + 2);
@@ -613,6 +616,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                 huOut = dhu;
                 hvOut = dhv;
             }
+            bad = bad || !(__builtin_isfinite(wOut) && __builtin_isfinite(huOut) && __builtin_isfinite(hvOut));
             st32(p.w_new, i, wOut);
             st32(p.Dx, i - 2u, huOut);
             dystg[y - ys][lane] = hvOut;
@@ -641,6 +645,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             row(y + 1);
         }
     for (; y < ye; ++y) row(y);
+    // one atomic per wave that saw a non-finite value
+    if (__ballot(bad) != 0ull && lane == 0 && p.status) atomicOr(p.status, 1);
     // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
     if (out_lane) {
         real *dst = p.DyT + (size_t)x * PT + (ys - 2);
@@ -660,7 +666,7 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
 }
 
 template <int SCHEME, bool COMP, int R, bool FAST>
-__global__ __launch_bounds__(64) void explicit_step(const ExplicitParams p)
+__global__ __launch_bounds__(64, 4) void explicit_step(const ExplicitParams p)
 {
     __shared__ ExplicitSmem<SCHEME == 2, R> sm;
     // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of

@@ -60,6 +60,9 @@ struct ExplicitParams {
     real *Dx;       // right-hand side of the x-line systems, [y * P + (x - 2)]
     real *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
     real *Gs[3], *FGs[2];
+    // numeric sentinel: bit 0 is raised when a new w or a line right-hand side is not finite
+    // (null: not checked)
+    int *status;
     // SimConstBuffer (compute.hlsl:65-94)
     real two_theta, g, half_g, Bcoef_g, g_over_dx, g_over_dy, one_over_dx, one_over_dy;
     real dt, epsilon, friction, sea_level;

@@ -268,6 +268,12 @@ typedef struct {
 
 /* GetStats of the current state (synchronises). */
 int cel_engine_stats(cel_engine *e, cel_stats *out);
+/* Numeric sentinel: CEL_ENUMERIC once a step has produced a non-finite surface elevation
+ * or line right-hand side (the explicit kernel raises a device flag; a non-finite value
+ * anywhere in the state reaches one of them within a step), else CEL_OK.  Waits for the
+ * engine's work; cel_engine_set_state clears it.  The reference has no such check
+ * (h >= 0 is kept only numerically, compute.hlsl:317-328, 637). */
+int cel_engine_check_numeric(cel_engine *e);
 
 /* resetTimestep (engine.cpp:1536-1756) without the GUI parts: GetStats, then the dt of
  * the next step -- adaptive (max(target/rate, timestep*min_ratio), moving average when
@@ -309,6 +315,15 @@ int cel_batch_timed_steps(cel_batch *b, int nsteps, float *ms);
 int cel_batch_count(cel_batch *b, int *n);
 int cel_batch_device_bytes(cel_batch *b, size_t *bytes);
 /* Per-scenario access (synchronises the batch first). */
+/* Hot start of one member: its state from state3 (as cel_engine_set_state), clearing its
+ * numeric sentinel.  No counterpart in the reference. */
+int cel_batch_set_state(cel_batch *b, int id, const float *state3);
+/* Numeric sentinel of the members: ids (up to max_ids of them) of the members whose steps
+ * produced a non-finite surface elevation or line right-hand side since they were added
+ * or last given a state, and their number.  Waits for the batch's work.  The other
+ * members are unaffected: a caller evicts the failed ones (cel_batch_remove).  No
+ * counterpart in the reference (which has no NaN check, engine.cpp:1536-1756). */
+int cel_batch_check_numeric(cel_batch *b, int *ids, int max_ids, int *n_failed);
 int cel_batch_get_state(cel_batch *b, int id, float *state3);
 int cel_batch_get_state_f64(cel_batch *b, int id, double *state3);
 int cel_batch_stats(cel_batch *b, int id, cel_stats *out);

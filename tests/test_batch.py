@@ -101,3 +101,56 @@ def test_batch_rejects_corrector():
     with pytest.raises(CeleriError):
         b.add(sc.s, sc.boundaries, sc.bottom, sc.state)
     b.close()
+
+
+def test_numeric_sentinel_engine():
+    """A non-finite value put into the state raises the engine's sentinel within a step; a
+    clean state clears it."""
+    sc = S.small(nx=48, ny=40, seed=11)
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(20)
+    assert e.numeric_ok()
+    st = e.get_state()
+    bad = st.copy()
+    bad[1, 20, 25] = np.nan
+    e.set_state(bad)
+    e.timestep(3)
+    assert not e.numeric_ok()
+    e.set_state(st)
+    assert e.numeric_ok()
+    e.timestep(5)
+    assert e.numeric_ok()
+    e.close()
+
+
+@pytest.mark.parametrize("graph_steps", [0, 10])
+def test_numeric_sentinel_flags_only_the_failed_member(graph_steps):
+    """NaN put into one member of a batch: only that member is reported, and the others end
+    bit-identical to their runs alone."""
+    scs = [S.small(nx=48, ny=40, seed=21), S.small(nx=90, ny=60, seed=22), S.small(nx=70, ny=130, seed=23)]
+    b = Batch(graph_steps=graph_steps)
+    ids = [b.add(sc.s, sc.boundaries, sc.bottom, sc.state) for sc in scs]
+    b.step(10)
+    assert b.failed() == []
+    states = [b.get_state(i) for i in ids]
+    bad = states[1].copy()
+    bad[0, 30, 40] = np.inf
+    for i, st in zip(ids, (states[0], bad, states[2])):
+        b.set_state(i, st)                       # hot start of every member (the scheme restarts)
+    b.step(20)
+    assert b.failed() == [ids[1]]
+    for k in (0, 2):
+        sc = scs[k]
+        e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        e.timestep(10)
+        e.set_state(e.get_state())
+        e.timestep(20)
+        ref = e.get_state()
+        e.close()
+        got = b.get_state(ids[k])
+        assert np.isfinite(got).all()
+        assert np.array_equal(got, ref)
+    b.remove(ids[1])
+    b.step(5)
+    assert b.failed() == []
+    b.close()
