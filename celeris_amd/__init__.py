@@ -129,6 +129,10 @@ SIGNATURES = {
     "cel_batch_check_numeric": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_batch_set_state": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
+    "cel_stats_noalloc": ([ctypes.POINTER(CelSettings), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "cel_stats_from_sums": ([ctypes.POINTER(CelSettings), ctypes.c_float, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(CelStats)], ctypes.c_int),
 }
 
 _lib = None

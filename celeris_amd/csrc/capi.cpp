@@ -16,6 +16,8 @@ void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
 void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3,
                    const float *base3);
 float probe_bandwidth(int kind, size_t bytes, int launches);
+hipError_t stats_noalloc(int nx, int ny, int pitch, const float *w, const float *hu, const float *hv, const float *BA,
+                         float g, float one_over_dx, float one_over_dy, float epsilon, double *out9, hipStream_t st);
 }
 
 struct cel_engine {
@@ -399,6 +401,35 @@ int cel_engine_reset_timestep(cel_engine *e, const cel_adaptive *a, cel_stats *o
         if (!(a->ema_alpha >= 0.f && a->ema_alpha <= 1.f)) throw cel::Error(CEL_EINVAL, "ema_alpha must lie in [0, 1]");
         if (!(a->min_timestep_ratio > 0.f)) throw cel::Error(CEL_EINVAL, "min_timestep_ratio must be positive");
         eng(e).resetTimestepAdaptive(*a, out);
+    });
+}
+
+int cel_stats_noalloc(const cel_settings *s, const float *w, const float *hu, const float *hv, const float *BA,
+                      int pitch, double *sums9, void *stream)
+{
+    return guarded([&] {
+        if (!s || !w || !hu || !hv || !BA || !sums9) throw cel::Error(CEL_EINVAL, "null argument");
+        if (s->nx < 4 || s->ny < 4 || pitch < s->nx + 4) throw cel::Error(CEL_EINVAL, "bad grid size or pitch");
+        if (!(s->width > 0.f) || !(s->length > 0.f) || !(s->gravity > 0.f))
+            throw cel::Error(CEL_EINVAL, "width, length and gravity must be positive");
+        // the engine's constants (fillConstantBuffers, engine.cpp:4099-4136; CalcEpsilon, engine.cpp:290-299)
+        const float dx = s->width / (float)(s->nx - 1), dy = s->length / (float)(s->ny - 1);
+        const float m = std::max(dx, dy);
+        const float eps = std::min(s->epsilon, (m * m) * (m * m));
+        const hipError_t e = cel::stats_noalloc(s->nx, s->ny, pitch, w, hu, hv, BA, s->gravity, 1.f / dx, 1.f / dy, eps,
+                                                sums9, static_cast<hipStream_t>(stream));
+        if (e == hipErrorInvalidValue)
+            throw cel::Error(CEL_EINVAL, "grid above the 4096 x 4096 statistics scratch (16384 tiles of 64 x 16 cells)");
+        if (e != hipSuccess) throw cel::Error(CEL_EDEVICE, std::string("stats: ") + hipGetErrorString(e));
+    });
+}
+
+int cel_stats_from_sums(const cel_settings *s, float dt, const double *sums9, cel_stats *out)
+{
+    return guarded([&] {
+        if (!s || !sums9 || !out) throw cel::Error(CEL_EINVAL, "null argument");
+        if (s->nx < 4 || s->ny < 4) throw cel::Error(CEL_EINVAL, "bad grid size");
+        cel::derive_stats(*s, (double)dt, sums9, out);
     });
 }
 

@@ -975,10 +975,10 @@ void ShallowWaterEngine::runStats(double *out9)
     StatsParams p;
     std::memset(&p, 0, sizeof(p));
     p.ncx = 4 * (nx_ / 4); p.ncy = 4 * (ny_ / 4);
-    p.P = P_; p.PT = PT_;
+    p.P = P_; p.hv_sx = PT_; p.hv_sy = 1;
     p.tiles_x = (p.ncx + kStatsTileX - 1) / kStatsTileX;
     p.nblocks = stats_blocks_;
-    p.w = w_[cur_]; p.hu = hu_[cur_]; p.hvT = hvT_[cur_]; p.BA = BA_;
+    p.w = w_[cur_]; p.hu = hu_[cur_]; p.hv = hvT_[cur_]; p.BA = BA_;
     p.g = s_.gravity;
     p.one_over_dx = real(1) / dx_;
     p.one_over_dy = real(1) / dy_;

@@ -21,6 +21,10 @@
 // real = float (namespace cel, the reference's precision) and, with CEL_FP64 defined,
 // real = double (namespace cel64, the FP64 variant).
 #include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+
 #include "params.hpp"
 
 
@@ -1376,11 +1380,11 @@ __global__ __launch_bounds__(256) void stats_partial(const StatsParams p)
     const int tid = threadIdx.x;
     const int tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
     const int x0 = 2 + tx * kStatsTileX, y0 = 2 + ty * kStatsTileY;
-    // hv tile through LDS: hvT is column-major, lanes walk y
+    // hv tile through LDS (the engine's hv is column-major: lanes walk y)
     for (int k = tid; k < kStatsTileX * kStatsTileY; k += 256) {
         const int xl = k / kStatsTileY, yl = k % kStatsTileY;
         const int x = x0 + xl, y = y0 + yl;
-        hvs[yl][xl] = (x < 2 + p.ncx && y < 2 + p.ncy) ? p.hvT[(size_t)x * p.PT + y] : 0.f;
+        hvs[yl][xl] = (x < 2 + p.ncx && y < 2 + p.ncy) ? p.hv[(size_t)x * p.hv_sx + (size_t)y * p.hv_sy] : 0.f;
     }
     __syncthreads();
     double sh = 0, sb = 0, shu = 0, shv = 0, sk = 0;
@@ -1884,6 +1888,58 @@ hipError_t launch_stats(const StatsParams &p, hipStream_t st)
     hipLaunchKernelGGL(stats_final, dim3(1), dim3(64), 0, st, p);
     return hipGetLastError();
 }
+
+#ifndef CEL_FP64
+// ------------------------------------------------------------------ statistics without allocation
+//
+// GetStats over caller-owned device planes (row-major, ghost-inclusive, one pitch) with the
+// per-block partial sums in a file-scope device array rather than engine memory, so the call
+// allocates nothing: the same stats_partial / stats_final launches as an engine's statistics
+// (same tiles, same orders), hence the same results.  Each GPU holds its own copy of the array
+// until the process ends; host code uses the address hipGetSymbolAddress reports for the
+// current device, looked up once per device.  Calls that overlap on one GPU share the
+// array and race; the partials need no clearing (every block writes its own).
+constexpr int kNoAllocStatsTiles = 16384;    // 64 x 16-cell tiles: grids up to 4096 x 4096
+static __device__ __align__(256) double scratch[(size_t)kNoAllocStatsTiles * kStatsValues];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_address[kMaxDevices] = {};  // per device; null until looked up
+}  // namespace
+
+hipError_t stats_noalloc(int nx, int ny, int pitch, const real *w, const real *hu, const real *hv, const real *BA,
+                         real g, real one_over_dx, real one_over_dy, real epsilon, double *out9, hipStream_t st)
+{
+    StatsParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.ncx = 4 * (nx / 4); p.ncy = 4 * (ny / 4);
+    p.tiles_x = (p.ncx + kStatsTileX - 1) / kStatsTileX;
+    p.nblocks = p.tiles_x * ((p.ncy + kStatsTileY - 1) / kStatsTileY);
+    if (p.nblocks > kNoAllocStatsTiles) return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    double *partial = nullptr;
+    {
+        if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (!g_scratch_address[dev]) {
+            void *addr = nullptr;
+            e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return e;
+            g_scratch_address[dev] = static_cast<double *>(addr);
+        }
+        partial = g_scratch_address[dev];
+    }
+    p.P = pitch; p.hv_sx = 1; p.hv_sy = pitch;
+    p.w = w; p.hu = hu; p.hv = hv; p.BA = BA;
+    p.g = g; p.one_over_dx = one_over_dx; p.one_over_dy = one_over_dy; p.epsilon = epsilon;
+    p.partial = partial;
+    p.out = out9;
+    return launch_stats(p, st);
+}
+#endif
 
 hipError_t launch_export_coefficients(const StaticParams &p, real *ax, real *ay, hipStream_t st)
 {

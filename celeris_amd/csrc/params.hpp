@@ -178,9 +178,10 @@ struct BoundaryJob {
 // kStatsTileX x kStatsTileY cells per block, nine partials per block.
 constexpr int kStatsTileX = 64, kStatsTileY = 16, kStatsValues = 9;
 struct StatsParams {
-    int ncx, ncy, P, PT;
+    int ncx, ncy, P;                 // reduced cells; row pitch of w, hu and BA
+    int hv_sx, hv_sy;                // hv element (x, y) at hv[x * hv_sx + y * hv_sy]
     int tiles_x, nblocks;
-    const real *w, *hu, *hvT, *BA;
+    const real *w, *hu, *hv, *BA;
     real g, one_over_dx, one_over_dy, epsilon;
     double *partial;   // [block * kStatsValues + v]
     double *out;       // kStatsValues

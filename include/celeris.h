@@ -268,6 +268,21 @@ typedef struct {
 
 /* GetStats of the current state (synchronises). */
 int cel_engine_stats(cel_engine *e, cel_stats *out);
+/* GetStats without allocation: the nine FP64 sums and maxima of cel_engine_stats (the same
+ * kernels, tiles and reduction orders, so the same bits) over caller-owned DEVICE planes
+ * w, hu, hv, BA (row-major, ghost-inclusive, (nx+4) x (ny+4) cells, `pitch` floats per
+ * row), written to the DEVICE array sums9 (9 doubles) on `stream` (a hipStream_t; NULL for
+ * the default stream), asynchronously.  Its partial sums live in a file-scope device array
+ * of 16384 tiles (1.2 MB per GPU, held until the process ends), so grids above
+ * 4096 x 4096 cells are rejected (CEL_EINVAL); the call makes no hipMalloc, hipFree or
+ * other allocating call, for callers that cannot allocate (HIP graph capture, serving
+ * loops, harnesses that forbid it).  Calls that overlap on one GPU share that array and
+ * race.  No counterpart in the reference. */
+int cel_stats_noalloc(const cel_settings *s, const float *w, const float *hu, const float *hv, const float *BA,
+                      int pitch, double *sums9, void *stream);
+/* resetTimestep's post-processing (engine.cpp:1686-1696) of nine host-side sums from
+ * cel_stats_noalloc, with dt the next step's: the cel_stats cel_engine_stats reports. */
+int cel_stats_from_sums(const cel_settings *s, float dt, const double *sums9, cel_stats *out);
 /* Numeric sentinel: CEL_ENUMERIC once a step has produced a non-finite surface elevation
  * or line right-hand side (the explicit kernel raises a device flag; a non-finite value
  * anywhere in the state reaches one of them within a step), else CEL_OK.  Waits for the
