@@ -106,29 +106,51 @@ def cpu_baselines(seconds_target=10.0):
 
 def serving_leg(n, rates, seed, block):
     """A bounded sample of the serving stream: n requests (generated before the clock
-    starts), served in real time with continuous batching at each arrival rate, and the
-    plain engine serving them one at a time (FIFO replay of each request's measured run
-    time alone).  The same seed gives the same requests at every rate; arrival times scale
-    with 1/rate."""
+    starts), served in real time with continuous batching with the whole backlog present at
+    t = 0 (saturated) and at each Poisson arrival rate, and the plain engine serving them one
+    at a time (FIFO replay of each request's measured time alone, setup included on both
+    sides).  The same seed gives the same requests at every rate; arrival times scale with
+    1/rate.  The bound is the largest ratio any batching could reach on this mix: the time
+    alone summed over the requests against their cell-steps at the best single-grid rate."""
+    from celeris_amd import ShallowWaterEngine
     from celeris_amd import scenarios as S
     from celeris_amd import serving
     base = S.serving_requests(n, 1.0, seed=seed)
+    warm = S.small(nx=32, ny=32)                     # the HIP runtime's first-use cost is nobody's setup
+    ShallowWaterEngine(warm.s, warm.boundaries, warm.bottom, warm.state).close()
     alone_times = {r.rid: serving.alone_run_time(r) for r in base}
+    big = max(base, key=lambda r: r.scenario.s.nx * r.scenario.s.ny)
+    e = ShallowWaterEngine(big.scenario.s, big.scenario.boundaries, big.scenario.bottom, big.scenario.state)
+    e.timestep(20)
+    e.sync()
+    t0 = time.perf_counter()
+    e.timestep(200)
+    e.sync()
+    best_rate = big.scenario.s.nx * big.scenario.s.ny * 200 / (time.perf_counter() - t0)
+    e.close()
     sides = {}
     for r in base:
         sides[r.scenario.s.nx] = sides.get(r.scenario.s.nx, 0) + 1
+    bound = serving.throughput_bound(base, alone_times, best_rate)
     out = {"workload": f"{n} requests, square grids {dict(sorted(sides.items()))}, 500-20000 steps (log-uniform), "
                        f"sloped seabed + GRF, sine/irregular forcing, compensated update, {block}-step blocks",
-           "rates": {}}
-    for rate in rates:
-        reqs = [S.Request(r.rid, r.arrival / rate, r.scenario, r.steps) for r in base]
+           "best_single_grid_gcell_steps_per_s": best_rate / 1e9, "throughput_ratio_bound": bound, "rates": {}}
+    for rate in ["saturated"] + list(rates):
+        if rate == "saturated":
+            reqs = [S.Request(r.rid, 0.0, r.scenario, r.steps) for r in base]
+            key = "saturated"
+        else:
+            reqs = [S.Request(r.rid, r.arrival / rate, r.scenario, r.steps) for r in base]
+            key = f"{rate:g}/s"
         _, batched, _ = serving.serve(reqs, block=block)
         _, srpt, _ = serving.serve(reqs, block=block, policy="srpt")
         _, alone = serving.one_at_a_time(reqs, run_times=alone_times)
-        out["rates"][f"{rate:g}/s"] = {
+        out["rates"][key] = {
             "batched": batched, "batched_srpt": srpt, "one_at_a_time": alone,
             "throughput_ratio": batched["gcell_steps_per_s"] / alone["gcell_steps_per_s"],
             "throughput_ratio_srpt": srpt["gcell_steps_per_s"] / alone["gcell_steps_per_s"]}
+        print(f"bench: serving {key}: batched/alone {out['rates'][key]['throughput_ratio']:.2f}, srpt "
+              f"{out['rates'][key]['throughput_ratio_srpt']:.2f} (bound {bound:.2f})", file=sys.stderr, flush=True)
     return out
 
 
@@ -263,7 +285,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
-    ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu", "bandwidth"], default="headline")
+    ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu", "bandwidth", "serving"], default="headline")
     ap.add_argument("--members", type=int, default=64, help="ensemble scenarios")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
@@ -289,6 +311,15 @@ def main():
         if "--nx" not in sys.argv:
             args.nx, args.ny = 4096, 1024
         print(json.dumps(precision_line(args)), flush=True)
+        return
+    if args.workload == "serving":
+        rates = [float(v) for v in args.serving_rates.split(",")]
+        leg = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)
+        sat = leg["rates"]["saturated"]
+        print(json.dumps({"metric": "serving throughput ratio (batched / one at a time, saturated)",
+                          "value": sat["throughput_ratio"], "unit": "x", "n_gpus": 1, "steps": 0, "warmup": 0,
+                          "higher_is_better": True, "vs_baseline": None, "dtype": "f32 + f32 (compensated)",
+                          "data": "synthetic", "config": {"workload": leg["workload"], "serving": leg}}), flush=True)
         return
     if args.workload == "bandwidth":
         print(json.dumps(bandwidth_line()), flush=True)

@@ -129,6 +129,7 @@ SIGNATURES = {
     "cel_batch_check_numeric": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_batch_set_state": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
+    "cel_batch_adopt": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_stats_noalloc": ([ctypes.POINTER(CelSettings), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "cel_stats_from_sums": ([ctypes.POINTER(CelSettings), ctypes.c_float, ctypes.POINTER(ctypes.c_double),
@@ -471,6 +472,16 @@ class Batch:
         i = ctypes.c_int()
         _check(load_library().cel_batch_add(self._h, ctypes.byref(s), cb, _fptr(bottom), _fptr(state), ctypes.byref(i)))
         self._dims[i.value] = (s.nx, s.ny)
+        return i.value
+
+    def adopt(self, engine):
+        """Moves a ShallowWaterEngine (FP32 or compensated, created anywhere, e.g. on another
+        thread while the batch runs) into the batch; returns its member id.  The engine
+        object is closed."""
+        i = ctypes.c_int()
+        _check(load_library().cel_batch_adopt(self._h, engine._h, ctypes.byref(i)))
+        self._dims[i.value] = (engine.nx, engine.ny)
+        engine.close()
         return i.value
 
     def remove(self, sid):

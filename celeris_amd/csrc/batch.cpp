@@ -47,11 +47,21 @@ int Batch::add(const cel_settings &s, const cel_boundary bnd[4], const float *bo
     if (s.correction_steps_num != 0) throw Error(CEL_EINVAL, "batched scenarios do not take corrector steps");
     if (s.state_precision == CEL_STATE_FP64) throw Error(CEL_EINVAL, "batched scenarios run in FP32 (plain or compensated)");
     if (free_slots_.empty()) throw Error(CEL_ENOMEM, "a batch holds at most 4096 scenarios");
-    sync();
+    // the new engine is set up on its own stream while the batch's launches run: nothing
+    // in flight touches it, and the next step rebuilds the work-lists
     auto e = std::make_unique<ShallowWaterEngine>(s, bnd, bottom3, state3);
+    return adopt(e);
+}
+
+int Batch::adopt(std::unique_ptr<ShallowWaterEngine> &e)
+{
+    if (!e) throw Error(CEL_EINVAL, "null engine");
+    if (e->settings().correction_steps_num != 0) throw Error(CEL_EINVAL, "batched scenarios do not take corrector steps");
+    if (free_slots_.empty()) throw Error(CEL_ENOMEM, "a batch holds at most 4096 scenarios");
+    e->sync();                                   // its uploads are complete before any batch step
     const int id = next_id_++;
     const int slot = free_slots_.back();
-    check(hipMemsetAsync(d_status_ + slot, 0, sizeof(int), stream_), "clear member status");
+    check(hipMemsetAsync(d_status_ + slot, 0, sizeof(int), stream_), "clear member status");   // ordered before its steps
     free_slots_.pop_back();
     slot_[id] = slot;
     scen_[id] = std::move(e);
@@ -339,7 +349,9 @@ void Batch::run(int m, bool use_graph)
             if (g.m == m) exec = g.exec;
         if (!exec) {
             hipGraph_t graph = nullptr;
-            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            // relaxed: other host threads may be creating engines (allocations, copies on their
+            // own streams) while this stream is captured
+            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
             try {
                 issue(d_blob_, stride_, m);
             } catch (...) {

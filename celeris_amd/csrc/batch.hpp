@@ -34,6 +34,9 @@ public:
     Batch &operator=(const Batch &) = delete;
 
     int add(const cel_settings &s, const cel_boundary bnd[4], const float *bottom3, const float *state3);
+    // takes over an engine created elsewhere (e.g. on another host thread while the batch
+    // runs); e is moved from only on success
+    int adopt(std::unique_ptr<ShallowWaterEngine> &e);
     void remove(int id);
     void setPaused(int id, bool paused);     // paused scenarios are skipped by step()
     void step(int nsteps);

@@ -336,6 +336,28 @@ int cel_batch_check_numeric(cel_batch *b, int *ids, int max_ids, int *n_failed)
     });
 }
 
+int cel_batch_adopt(cel_batch *b, cel_engine *e, int *id)
+{
+    return guarded([&] {
+        cel::Batch &bt = bat(b);
+        if (!e || !e->impl) throw cel::Error(CEL_EINVAL, "null engine");
+        auto *fp32 = dynamic_cast<cel::ShallowWaterEngine *>(e->impl);
+        if (!fp32) throw cel::Error(CEL_EINVAL, "batched scenarios run in FP32 (plain or compensated)");
+        if (fp32->settings().correction_steps_num != 0)
+            throw cel::Error(CEL_EINVAL, "batched scenarios do not take corrector steps");
+        std::unique_ptr<cel::ShallowWaterEngine> up(fp32);
+        int v = 0;
+        try {
+            v = bt.adopt(up);
+        } catch (...) {
+            up.release();                        // still the caller's
+            throw;
+        }
+        e->impl = nullptr;                       // the batch owns it from here on
+        if (id) *id = v;
+    });
+}
+
 int cel_batch_set_state(cel_batch *b, int id, const float *state3)
 {
     return guarded([&] {

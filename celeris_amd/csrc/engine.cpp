@@ -189,11 +189,10 @@ void ShallowWaterEngine::uploadHostBottom()
 {
     const size_t N = (size_t)X_ * Y_;
     const size_t e = sizeof(real);
+    (void)e;
     const std::vector<real> &b = hostBottom_;
     real *dst[3] = {BY_, BX_, BA_};
-    for (int c = 0; c < 3; ++c)
-        check(hipMemcpy2DAsync(dst[c], (size_t)P_ * e, b.data() + c * N, (size_t)X_ * e, (size_t)X_ * e, Y_,
-                               hipMemcpyHostToDevice, stream_), "upload bottom");
+    for (int c = 0; c < 3; ++c) stagePlane(dst[c], b.data() + c * N);
     hostBA_.assign(b.begin() + 2 * N, b.begin() + 3 * N);
     StaticParams sp;
     fillStatic(sp);
@@ -203,6 +202,20 @@ void ShallowWaterEngine::uploadHostBottom()
     check(hipMemcpyAsync(&unsafe, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read flag");
     check(hipStreamSynchronize(stream_), "upload sync");
     fast_mesh_ = unsafe == 0 && fastSettings();
+}
+
+void ShallowWaterEngine::stagePlane(real *dst, const real *src)
+{
+    const size_t e = sizeof(real);
+    check(hipMemcpyAsync(Dx_, src, (size_t)X_ * Y_ * e, hipMemcpyHostToDevice, stream_), "upload plane");
+    check(hipMemcpy2DAsync(dst, (size_t)P_ * e, Dx_, (size_t)X_ * e, (size_t)X_ * e, Y_, hipMemcpyDeviceToDevice,
+                           stream_), "pitch plane");
+}
+
+void ShallowWaterEngine::stagePlaneT(real *dstT, const real *src)
+{
+    check(hipMemcpyAsync(Dx_, src, (size_t)X_ * Y_ * sizeof(real), hipMemcpyHostToDevice, stream_), "upload plane");
+    check(launch_transpose(Dx_, X_, dstT, PT_, Y_, X_, stream_), "transpose plane");
 }
 
 // The explicit kernel's FAST variant needs: FP32 state, the Boussinesq coefficients (their
@@ -239,22 +252,22 @@ void ShallowWaterEngine::setState(const float *state3)
     check(hipStreamSynchronize(stream_), "sync");
     const size_t N = (size_t)X_ * Y_;
     const size_t e = sizeof(real);
-    const std::vector<real> st(state3, state3 + 2 * N);
-    for (int k = 0; k < 2; ++k) {
-        check(hipMemcpy2DAsync(w_[k], (size_t)P_ * e, st.data(), (size_t)X_ * e, (size_t)X_ * e, Y_,
-                               hipMemcpyHostToDevice, stream_), "upload w");
-        check(hipMemcpy2DAsync(hu_[k], (size_t)P_ * e, st.data() + N, (size_t)X_ * e, (size_t)X_ * e, Y_,
-                               hipMemcpyHostToDevice, stream_), "upload hu");
+    // the state in the working precision (the caller's planes themselves in FP32); both
+    // ping-pong buffers receive it
+    std::vector<real> conv;
+    const real *st = reinterpret_cast<const real *>(state3);
+    if (sizeof(real) != sizeof(float)) {
+        conv.assign(state3, state3 + 3 * N);
+        st = conv.data();
     }
-    std::vector<real> t(N);
-    const float *hv = state3 + 2 * N;
-    for (int y = 0; y < Y_; ++y)
-        for (int x = 0; x < X_; ++x) t[(size_t)x * Y_ + y] = hv[(size_t)y * X_ + x];
-    for (int k = 0; k < 2; ++k)
-        check(hipMemcpy2DAsync(hvT_[k], (size_t)PT_ * e, t.data(), (size_t)Y_ * e, (size_t)Y_ * e, X_,
-                               hipMemcpyHostToDevice, stream_), "upload hv");
+    const size_t plane = (size_t)P_ * Y_ * e, tplane0 = (size_t)PT_ * X_ * e;
+    stagePlane(w_[0], st);
+    stagePlane(hu_[0], st + N);
+    stagePlaneT(hvT_[0], st + 2 * N);
+    check(hipMemcpyAsync(w_[1], w_[0], plane, hipMemcpyDeviceToDevice, stream_), "copy w");
+    check(hipMemcpyAsync(hu_[1], hu_[0], plane, hipMemcpyDeviceToDevice, stream_), "copy hu");
+    check(hipMemcpyAsync(hvT_[1], hvT_[0], tplane0, hipMemcpyDeviceToDevice, stream_), "copy hv");
     // history textures start at zero (engine.cpp:3024-3034 creates them without data)
-    const size_t plane = (size_t)P_ * Y_ * e;
     for (auto &g : G_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
     for (auto &g : FG_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
     if (comp_) {
@@ -876,7 +889,7 @@ void ShallowWaterEngine::graphBlock()
         if (g.key == key) exec = g.exec;
     if (!exec) {
         hipGraph_t graph = nullptr;
-        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
         try {
             for (int k = 0; k < K; ++k) issueStep(plans[k], d_clock_ + (size_t)k * kClockStride, false);
         } catch (...) {

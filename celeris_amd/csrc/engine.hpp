@@ -37,6 +37,8 @@ int default_shape(int n);
 int shape_capacity(int shape);
 hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st);
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st);
+// out[j * pout + i] = in[i * pin + j] for i < rows, j < cols
+hipError_t launch_transpose(const real *in, int pin, real *out, int pout, int rows, int cols, hipStream_t st);
 hipError_t launch_export_coefficients(const StaticParams &p, real *ax, real *ay, hipStream_t st);
 hipError_t launch_stats(const StatsParams &p, hipStream_t st);
 hipError_t launch_gather(const real *w, const real *hu, const real *hvT, int P, int PT, const int *xy, int n,
@@ -104,6 +106,7 @@ public:
     bool kernelTime(const std::string &name, float *ms, int *launches) override;
     size_t deviceBytes() const override { return arena_bytes_ + table_bytes_; }
     bool compensated() const { return comp_; }
+    const cel_settings &settings() const { return s_; }
     // the explicit kernel's fast variant (shortened exact division and square root) applies
     bool fastArith() const { return fast_mesh_ && fast_allowed_; }
     // device bytes an engine with these settings holds after creation (admission control)
@@ -143,6 +146,11 @@ private:
     void allocate();
     void uploadBottom(const float *bottom3);
     void uploadHostBottom();
+    // one host plane of X x Y values (contiguous) into a device plane: one contiguous upload
+    // into the x-line right-hand-side buffer (unused between steps), then a pitched copy
+    // (stagePlane) or a transpose (stagePlaneT, column pitch PT) on the device
+    void stagePlane(real *dst, const real *src);
+    void stagePlaneT(real *dstT, const real *src);
     void bottomChanged();
     void downloadW(std::vector<real> &w);
     void uploadW(const std::vector<real> &w);

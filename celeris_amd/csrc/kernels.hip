@@ -1839,6 +1839,13 @@ hipError_t launch_boundary(const BoundaryParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_transpose(const real *in, int pin, real *out, int pout, int rows, int cols, hipStream_t st)
+{
+    hipLaunchKernelGGL(transpose_plane, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(32, 8), 0, st, in, pin, out, pout,
+                       rows, cols);
+    return hipGetLastError();
+}
+
 hipError_t launch_static_fields(const StaticParams &p, hipStream_t st)
 {
     const int X = p.nx + 4, Y = p.ny + 4;

@@ -330,6 +330,11 @@ int cel_batch_timed_steps(cel_batch *b, int nsteps, float *ms);
 int cel_batch_count(cel_batch *b, int *n);
 int cel_batch_device_bytes(cel_batch *b, size_t *bytes);
 /* Per-scenario access (synchronises the batch first). */
+/* Moves an engine (FP32 or compensated, no corrector steps) into the batch as a new member,
+ * *id its id: the engine can be created on another host thread while the batch runs.  On
+ * success the handle is left empty (cel_engine_destroy still frees it); on failure it
+ * stays the caller's.  No counterpart in the reference. */
+int cel_batch_adopt(cel_batch *b, cel_engine *e, int *id);
 /* Hot start of one member: its state from state3 (as cel_engine_set_state), clearing its
  * numeric sentinel.  No counterpart in the reference. */
 int cel_batch_set_state(cel_batch *b, int id, const float *state3);
