@@ -130,6 +130,8 @@ SIGNATURES = {
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cel_batch_set_state": ([ctypes.c_void_p, ctypes.c_int, _F], ctypes.c_int),
     "cel_batch_adopt": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cel_engine_checkpoint": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "cel_engine_restore": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
     "cel_stats_noalloc": ([ctypes.POINTER(CelSettings), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "cel_stats_from_sums": ([ctypes.POINTER(CelSettings), ctypes.c_float, ctypes.POINTER(ctypes.c_double),
@@ -363,6 +365,19 @@ class ShallowWaterEngine:
         ms = ctypes.c_float()
         _check(load_library().cel_engine_timed_steps(self._h, n, ctypes.byref(ms)))
         return ms.value
+
+    def checkpoint(self):
+        """Everything the next steps read, as a byte array (see cel_engine_checkpoint)."""
+        n = ctypes.c_size_t()
+        _check(load_library().cel_engine_checkpoint(self._h, None, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _check(load_library().cel_engine_checkpoint(self._h, buf.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+        return buf
+
+    def restore(self, buf):
+        """Continues from a checkpoint of an engine with the same inputs, bit-identically."""
+        buf = np.ascontiguousarray(buf, np.uint8)
+        _check(load_library().cel_engine_restore(self._h, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes))
 
     def numeric_ok(self):
         """False once a step has produced a non-finite value (the device sentinel); waits for

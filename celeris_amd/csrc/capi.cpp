@@ -318,6 +318,24 @@ int cel_batch_device_bytes(cel_batch *b, size_t *bytes)
     });
 }
 
+int cel_engine_checkpoint(cel_engine *e, void *buf, size_t *bytes)
+{
+    return guarded([&] {
+        if (!bytes) throw cel::Error(CEL_EINVAL, "null size");
+        const size_t need = eng(e).checkpoint(buf, buf ? *bytes : 0);
+        if (buf && *bytes < need) {
+            *bytes = need;
+            throw cel::Error(CEL_EINVAL, "checkpoint buffer too small");
+        }
+        *bytes = need;
+    });
+}
+
+int cel_engine_restore(cel_engine *e, const void *buf, size_t bytes)
+{
+    return guarded([&] { eng(e).restore(buf, bytes); });
+}
+
 int cel_engine_check_numeric(cel_engine *e)
 {
     return guarded([&] {

@@ -284,6 +284,110 @@ void ShallowWaterEngine::setState(const float *state3)
     resetScheme();
 }
 
+// ---- checkpoints
+//
+// Layout: a header (magic, version, precision, grid, pitches, plane count, scalar count),
+// the host scalars of the step state machine, then every device plane the next steps can
+// read, copied whole (pitch included) in checkpointPlanes' order.  The reference's hot start
+// (engine.cpp:2943-2970) keeps only (w, hu, hv) and restarts the scheme with Euler steps;
+// this keeps the Adams-Bashforth ring too.
+
+namespace {
+constexpr unsigned kCheckpointMagic = 0x43454c4bu;   // "CELK"
+constexpr int kCheckpointVersion = 1;
+struct CheckpointHeader {
+    unsigned magic;
+    int version, real_bytes, comp, nx, ny, P, PT, nplanes, nscalars, correction_steps, aux;
+};
+}  // namespace
+
+std::vector<std::pair<real *, size_t>> ShallowWaterEngine::checkpointPlanes() const
+{
+    const size_t plane = (size_t)P_ * Y_ * sizeof(real), tplane = (size_t)PT_ * X_ * sizeof(real);
+    std::vector<std::pair<real *, size_t>> v;
+    for (int k = 0; k < 2; ++k) { v.emplace_back(w_[k], plane); v.emplace_back(hu_[k], plane); v.emplace_back(hvT_[k], tplane); }
+    if (comp_)
+        for (int k = 0; k < 2; ++k) {
+            v.emplace_back(w_lo_[k], plane); v.emplace_back(hu_lo_[k], plane); v.emplace_back(hvT_lo_[k], tplane);
+        }
+    for (auto &g : G_) for (auto *c : g) v.emplace_back(c, plane);
+    for (auto &g : FG_) for (auto *c : g) v.emplace_back(c, plane);
+    if (s_.correction_steps_num > 0) {
+        for (auto *c : Gp_) v.emplace_back(c, plane);
+        for (auto *c : FGp_) v.emplace_back(c, plane);
+        v.emplace_back(Cw_, plane); v.emplace_back(Chu_, plane); v.emplace_back(ChvT_, tplane);
+    }
+    if (aux_on_) v.emplace_back(aux_mem_, 3 * plane);
+    return v;
+}
+
+size_t ShallowWaterEngine::checkpoint(void *buf, size_t cap)
+{
+    const auto planes = checkpointPlanes();
+    const double scal[] = {(double)cur_, (double)ring_old_, (double)ring_oo_, (double)ring_scr_,
+                           (double)time_scheme_now_, (double)timestep_count_, (double)correction_steps_count_,
+                           (double)total_time_, (double)current_timestep_, (double)old_timestep_,
+                           (double)old_old_timestep_, (double)side_[0].uts_iter, (double)side_[1].uts_iter,
+                           (double)side_[2].uts_iter, (double)side_[3].uts_iter, (double)uniform_dt_,
+                           (double)dt_initialized_, (double)target_cfl_set_, (double)target_cfl_};
+    constexpr int nscal = sizeof(scal) / sizeof(scal[0]);
+    size_t bytes = sizeof(CheckpointHeader) + sizeof(scal);
+    for (const auto &pl : planes) bytes += pl.second;
+    if (!buf || cap < bytes) return bytes;
+    sync();
+    char *q = static_cast<char *>(buf);
+    const CheckpointHeader h{kCheckpointMagic, kCheckpointVersion, (int)sizeof(real), comp_ ? 1 : 0, nx_, ny_, P_, PT_,
+                             (int)planes.size(), nscal, s_.correction_steps_num, aux_on_ ? 1 : 0};
+    std::memcpy(q, &h, sizeof(h));
+    q += sizeof(h);
+    std::memcpy(q, scal, sizeof(scal));
+    q += sizeof(scal);
+    for (const auto &pl : planes) {
+        check(hipMemcpyAsync(q, pl.first, pl.second, hipMemcpyDeviceToHost, stream_), "checkpoint copy");
+        q += pl.second;
+    }
+    sync();
+    return bytes;
+}
+
+void ShallowWaterEngine::restore(const void *buf, size_t bytes)
+{
+    if (!buf || bytes < sizeof(CheckpointHeader)) throw Error(CEL_EINVAL, "checkpoint too short");
+    CheckpointHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != kCheckpointMagic || h.version != kCheckpointVersion) throw Error(CEL_EINVAL, "not a checkpoint");
+    if (h.real_bytes != (int)sizeof(real) || h.comp != (comp_ ? 1 : 0) || h.nx != nx_ || h.ny != ny_ || h.P != P_ ||
+        h.PT != PT_ || h.correction_steps != s_.correction_steps_num)
+        throw Error(CEL_EINVAL, "checkpoint of an engine with another grid, precision or corrector setting");
+    if ((h.aux != 0) != aux_on_)
+        throw Error(CEL_EINVAL, "checkpoint visual outputs differ: set cel_engine_set_visual as when it was taken");
+    const auto planes = checkpointPlanes();
+    constexpr int nscal = 19;
+    size_t need = sizeof(CheckpointHeader) + nscal * sizeof(double);
+    for (const auto &pl : planes) need += pl.second;
+    if (h.nplanes != (int)planes.size() || h.nscalars != nscal || bytes != need)
+        throw Error(CEL_EINVAL, "checkpoint size does not match this engine");
+    const char *q = static_cast<const char *>(buf) + sizeof(h);
+    double scal[nscal];
+    std::memcpy(scal, q, sizeof(scal));
+    q += sizeof(scal);
+    sync();
+    clearGraphs();
+    for (const auto &pl : planes) {
+        check(hipMemcpyAsync(pl.first, q, pl.second, hipMemcpyHostToDevice, stream_), "restore copy");
+        q += pl.second;
+    }
+    check(hipMemsetAsync(d_status_, 0, sizeof(int), stream_), "clear status");
+    sync();
+    cur_ = (int)scal[0]; ring_old_ = (int)scal[1]; ring_oo_ = (int)scal[2]; ring_scr_ = (int)scal[3];
+    time_scheme_now_ = (int)scal[4]; timestep_count_ = (int)scal[5]; correction_steps_count_ = (int)scal[6];
+    total_time_ = (real)scal[7]; current_timestep_ = (real)scal[8]; old_timestep_ = (real)scal[9];
+    old_old_timestep_ = (real)scal[10];
+    for (int k = 0; k < 4; ++k) side_[k].uts_iter = (int)scal[11 + k];
+    uniform_dt_ = (float)scal[15];
+    dt_initialized_ = scal[16] != 0.0; target_cfl_set_ = scal[17] != 0.0; target_cfl_ = (float)scal[18];
+}
+
 int ShallowWaterEngine::numericStatus()
 {
     int v = 0;

@@ -130,6 +130,11 @@ public:
     // the numeric sentinel: nonzero once a step produced a non-finite value (waits for the
     // stream); setState clears it
     int numericStatus() override;
+    // everything the next steps read (state, history ring, corrector copies, visual record,
+    // scheme counters, dt history, time, time-series cursors) as one buffer: writes it into
+    // buf (if buf and cap suffice) and returns its size; restore continues bit-identically
+    size_t checkpoint(void *buf, size_t cap) override;
+    void restore(const void *buf, size_t bytes) override;
     int *statusWord() const { return d_status_; }
     void sample(const int *xy, int n, float *out3) override;
     int timestepCount() const override { return timestep_count_; }
@@ -162,6 +167,8 @@ private:
     void fillLines(LineParams &p) const;
     void fillStatic(StaticParams &p) const;
     void check(hipError_t e, const char *what) const;
+    // the device planes a checkpoint holds, in its order
+    std::vector<std::pair<real *, size_t>> checkpointPlanes() const;
     void issueStep(const StepLaunch &L, const real *dclock, bool profile);
 
     // captured multi-step blocks

@@ -78,6 +78,8 @@ public:
     virtual void shiftTerrain(float shift_up) = 0;
     virtual void editWater(float world_x, float world_y, float radius, float disp_A, float disp_B) = 0;
     virtual int numericStatus() = 0;
+    virtual size_t checkpoint(void *buf, size_t cap) = 0;
+    virtual void restore(const void *buf, size_t bytes) = 0;
 };
 
 }  // namespace cel

@@ -81,14 +81,14 @@ def cfl_dt(dx, hmax, cfl=DEFAULT_CFL):
     return float(np.float32(cfl * dx / np.sqrt(G * hmax)))
 
 
-def plain_beach(nx=256, ny=256, dx=0.5, steps=2000):
+def plain_beach(nx=256, ny=256, dx=0.5, steps=2000, cfl=DEFAULT_CFL):
     """Flat 1.0 m depth for x < 60 % of the domain, then a 1:20 slope up to a 0.3 m dry beach
     plateau; west sine wave (a 0.05 m, T 2 s), north/south walls, 20-cell east sponge,
-    Bcoef 1/15, minmod theta 1.3, dt from CFL 0.1 on the 1 m depth."""
+    Bcoef 1/15, minmod theta 1.3, dt from the CFL number (0.1) on the 1 m depth."""
     W, L = dx * (nx - 1), dx * (ny - 1)
     X, _ = corner_coords(nx, ny, W, L)
     z = np.where(X < 0.6 * W, -1.0, np.minimum(0.3, -1.0 + (X - 0.6 * W) / 20.0))
-    s = settings(nx, ny, W, L, cfl_dt(dx, 1.0), theta=1.3)
+    s = settings(nx, ny, W, L, cfl_dt(dx, 1.0, cfl), theta=1.3)
     bnd = [Boundary(SINEWAVE, 2, 0.0, amplitude=0.05, period=2.0, theta=0.0),
            Boundary(SPONGE, 20), Boundary(SOLID, 2), Boundary(SOLID, 2)]
     return _finish("plain_beach", s, bnd, z, steps)

@@ -268,6 +268,17 @@ typedef struct {
 
 /* GetStats of the current state (synchronises). */
 int cel_engine_stats(cel_engine *e, cel_stats *out);
+/* Checkpoint and resume.  cel_engine_checkpoint writes into buf (*bytes its capacity) all
+ * that the next steps read: both state buffers (and a compensated state's low words), the
+ * Adams-Bashforth ring of dU/dt and (F*, G*), the corrector's copies, the visual record,
+ * the scheme counters, the dt history, the time and the time-series cursors; with buf NULL
+ * it only sets *bytes to the size.  cel_engine_restore loads such a buffer into an engine
+ * created with the same settings, boundaries and bottom (and the same visual setting); it
+ * then continues bit-identically to the engine the checkpoint was taken from.  The
+ * reference's hot start (engine.cpp:2943-2970, cel_engine_set_state here) keeps only
+ * (w, hu, hv) and restarts the scheme with two Euler steps (engine.cpp:3267-3269). */
+int cel_engine_checkpoint(cel_engine *e, void *buf, size_t *bytes);
+int cel_engine_restore(cel_engine *e, const void *buf, size_t bytes);
 /* GetStats without allocation: the nine FP64 sums and maxima of cel_engine_stats (the same
  * kernels, tiles and reduction orders, so the same bits) over caller-owned DEVICE planes
  * w, hu, hv, BA (row-major, ghost-inclusive, (nx+4) x (ny+4) cells, `pitch` floats per

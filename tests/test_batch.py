@@ -154,3 +154,22 @@ def test_numeric_sentinel_flags_only_the_failed_member(graph_steps):
     b.step(5)
     assert b.failed() == []
     b.close()
+
+
+def test_adopted_engine_continues_in_the_batch():
+    """An engine stepped on its own and then moved into a batch (cel_batch_adopt) continues
+    exactly as if it had stayed alone."""
+    sc = S.small(nx=90, ny=70, seed=31)
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(12)
+    b = Batch()
+    other = S.small(nx=50, ny=40, seed=32)
+    b.add(other.s, other.boundaries, other.bottom, other.state)
+    sid = b.adopt(e)
+    b.step(25)
+    got = b.get_state_f64(sid)
+    b.close()
+    ref = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    ref.timestep(37)
+    assert np.array_equal(got, ref.get_state_f64())
+    ref.close()
