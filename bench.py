@@ -8,11 +8,14 @@ stencil, x- and y-line tridiagonal solves, boundaries) over the whole grid; inpu
 resident in HBM before the timed region.
 
 Prints one JSON line: Gcell-steps/s (interior cells x steps / wall time), the dominant
-kernel's achieved HBM bandwidth against MI355X's 8 TB/s, the CPU oracle timed on a
-bounded sample of the same workload, and a bounded sample of the serving stream (Poisson
-arrivals of scenarios with sides 256-2048 and 500-20000 steps, continuous batching with
-all resident requests advancing, and with shortest-remaining-work-first) with p50/p99
-completion latency beside the plain engine serving the same stream one request at a time.
+kernel's achieved HBM bandwidth against MI355X's 8 TB/s (and the library's own streaming
+probes on the same GPU), the CPU oracle timed on a bounded sample of the same workload, a
+bounded sample of the serving stream (scenarios with sides 256-2048 and 500-20000 steps,
+the whole backlog at once and Poisson arrivals, continuous batching with all resident
+requests advancing and with shortest-remaining-work-first) with p50/p99 completion latency
+beside the plain engine serving the same stream one request at a time, the batched
+ensemble (64 members of 128-1024 per side) against the same members one at a time, and the
+plain 256^2 beach over its 2000 steps with its CPU-oracle baseline.
 
 --workload ensemble prints the batched-ensemble line instead (64 scenarios with sides
 log-uniform in [128, 1024], compensated update, 50-step graph blocks) against the same
@@ -176,6 +179,80 @@ def bandwidth_line():
                                           "peak_gb_per_s": HBM_PEAK_GBS}}
 
 
+def ensemble_leg(n=64, steps=200, warmup=20, graph_steps=50):
+    """The batched ensemble (n members, sides log-uniform in [128, 1024], compensated update) in
+    captured blocks against the same members run one at a time with the plain engine: both
+    sides time the same steps after the same warm-up, setup excluded from both."""
+    from celeris_amd import Batch, ShallowWaterEngine
+    from celeris_amd import scenarios as S
+    members = S.ensemble(n, seed=4, steps=steps)
+    cells = sum(m.s.nx * m.s.ny for m in members)
+    b = Batch(graph_steps=graph_steps)
+    for m in members:
+        b.add(m.s, m.boundaries, m.bottom, m.state)
+    b.step(warmup)
+    rows = b.autotune(5, tune_lines=True)
+    b.step(graph_steps)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    batched = cells * steps / (time.perf_counter() - t0)
+    ok = b.failed() == []
+    b.close()
+    alone_s = 0.0
+    for m in members:
+        e = ShallowWaterEngine(m.s, m.boundaries, m.bottom, m.state)
+        e.timestep(warmup)
+        e.sync()
+        t0 = time.perf_counter()
+        e.timestep(steps)
+        e.sync()
+        alone_s += time.perf_counter() - t0
+        e.close()
+    alone = cells * steps / alone_s
+    return {"workload": f"{n} scenarios, sides log-uniform in [128, 1024] ({cells} cells), compensated update, "
+                        f"{graph_steps}-step captured blocks, {steps} timed steps",
+            "batched_gcell_steps_per_s": batched / 1e9, "one_at_a_time_gcell_steps_per_s": alone / 1e9,
+            "speedup": batched / alone, "band_rows": rows, "all_finite": ok}
+
+
+def plain_leg(steps=2000):
+    """The plain 256^2 beach for its 2000 steps on the GPU, and its CPU baseline: the FP32 oracle
+    (OpenMP on the host's cores) over the same steps, which also checks the GPU's result."""
+    from celeris_amd import ShallowWaterEngine
+    from celeris_amd import scenarios as S
+    from oracle.binding import Oracle, get_threads
+    sc = S.plain_beach(steps=steps)
+    cells = sc.s.nx * sc.s.ny
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(20)
+    e.sync()
+    ms = e.timed_steps(steps)
+    e.close()
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(steps)
+    a = e.get_state()
+    e.close()
+    o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
+    t0 = time.perf_counter()
+    o.step(steps)
+    cpu_s = time.perf_counter() - t0
+    b = o.state().astype(np.float64)
+    a = a.astype(np.float64)
+    # max |difference| per field over the oracle's scale (eta: max |eta|; hu, hv: the larger
+    # momentum component), and the wet cells (h > 1e-4 m) of both
+    mom = max(np.abs(b[1]).max(), np.abs(b[2]).max())
+    err = {"eta": float(np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()),
+           "hu": float(np.abs(a[1] - b[1]).max() / mom), "hv": float(np.abs(a[2] - b[2]).max() / mom)}
+    wet = lambda st: (st[0] - sc.bottom[2]) > 1e-4
+    return {"workload": f"plain beach {sc.s.nx}x{sc.s.ny}, dx 0.5 m, west sine, walls, east sponge, {steps} steps",
+            "gpu_gcell_steps_per_s": cells * steps / (ms * 1e-3) / 1e9,
+            "cpu_oracle_gcell_steps_per_s": cells * steps / cpu_s / 1e9, "cpu_threads": get_threads(),
+            "relative_error_vs_oracle": err,
+            "same_wet_dry_cells": bool((wet(a) == wet(b)).all())}
+
+
 def ensemble_line(args):
     from celeris_amd import Batch, ShallowWaterEngine
     from celeris_amd import scenarios as S
@@ -290,6 +367,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample of the headline line (0: skip)")
     ap.add_argument("--serving-rates", type=str, default="5,50", help="Poisson arrival rates (1/s), comma-separated")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the batched-ensemble leg of the headline line (0: skip)")
     ap.add_argument("--probe", choices=["read", "write", "copy"], default=None,
                     help="only run the library's HBM streaming probe of this kind on a 1 GiB buffer (4 launches: the "
                          "known-size reference for the HBM byte counters)")
@@ -409,11 +488,20 @@ def main():
         "cpu_baseline": None,
     }
     print(f"bench: {sc.s.nx}x{sc.s.ny} {args.precision} step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
+    out["device_bandwidth_gb_per_s"] = device_bandwidth()
+    print(f"bench: probes {out['device_bandwidth_gb_per_s']}", file=sys.stderr, flush=True)
     if args.serving_requests > 0:
         rates = [float(v) for v in args.serving_rates.split(",")]
         out["serving"] = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)
         print("bench: serving sample done", file=sys.stderr, flush=True)
+    if args.ensemble_members > 0:
+        out["ensemble"] = ensemble_leg(args.ensemble_members)
+        print(f"bench: ensemble {out['ensemble']['batched_gcell_steps_per_s']:.2f} Gcell-steps/s batched, "
+              f"{out['ensemble']['speedup']:.2f}x one at a time", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
+        out["plain_256"] = plain_leg()
+        print(f"bench: plain 256^2 {out['plain_256']['gpu_gcell_steps_per_s']:.2f} Gcell-steps/s", file=sys.stderr,
+              flush=True)
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)
 
