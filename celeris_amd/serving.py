@@ -6,9 +6,10 @@ arrived requests are admitted while their device footprint fits the HBM budget (
 first served; least work first under "srpt") and finished ones retire, so requests that
 arrive while others run join the running batch.  Admission overlaps the GPU: a block is
 launched (asynchronously), then at most one waiting request is set up on the host (its
-engine, tables and uploads) while that block runs, then the loop waits for the block.  A request's latency is its completion time
-minus its arrival time; a member whose state turns non-finite (the kernels' numeric
-sentinel) is evicted and recorded as failed, the others continue.
+engine, tables and uploads) while that block runs, then the loop waits for the block.  A
+request's latency is its completion time minus its arrival time; a member whose state turns
+non-finite (the kernels' numeric sentinel) is evicted and recorded as failed, the others
+continue.
 
 Two policies decide which resident requests advance in a block.  "fair" advances all of
 them (processor sharing).  "srpt" advances the requests with the least remaining work
