@@ -669,8 +669,15 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
 }
 
+// FP32: 4 waves per SIMD (at most 128 VGPRs), the stencil's occupancy; the FP64 variants
+// need more registers than that and take what they need instead of spilling
+#ifdef CEL_FP64
+#define CEL_EXPLICIT_BOUNDS __launch_bounds__(64)
+#else
+#define CEL_EXPLICIT_BOUNDS __launch_bounds__(64, 4)
+#endif
 template <int SCHEME, bool COMP, int R, bool FAST>
-__global__ __launch_bounds__(64, 4) void explicit_step(const ExplicitParams p)
+__global__ CEL_EXPLICIT_BOUNDS void explicit_step(const ExplicitParams p)
 {
     __shared__ ExplicitSmem<SCHEME == 2, R> sm;
     // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of

@@ -115,3 +115,110 @@ def test_paused_scenario_resumes_unchanged():
     assert np.array_equal(s0, e.get_state_f64()) and t0 == pytest.approx(e.time, rel=0, abs=0)
     assert t1 > t0
     e.close()
+
+
+class _FakeEngine:
+    """Stand-in for a resident scenario (CPU only): records its steps."""
+
+    def __init__(self, s, boundaries, bottom, state):
+        self.nx, self.ny = s.nx, s.ny
+        self.bad = not np.isfinite(state).all()
+
+
+class _FakeBatch:
+    """Stand-in for cel_batch on the CPU: members advance by counting; a member whose initial
+    state is not finite is reported by failed() after its first step."""
+
+    def __init__(self, graph_steps=0):
+        self.m, self.n, self.paused, self.log, self.stepped = {}, 1, set(), [], set()
+
+    def add(self, s, boundaries, bottom, state):
+        i = self.n
+        self.n += 1
+        self.m[i] = _FakeEngine(s, boundaries, bottom, state)
+        return i
+
+    def set_paused(self, i, p):
+        (self.paused.add if p else self.paused.discard)(i)
+
+    def step(self, k):
+        run = sorted(i for i in self.m if i not in self.paused)
+        self.log.append((k, run))
+        self.stepped.update(run)
+
+    def sync(self):
+        pass
+
+    def failed(self):
+        return [i for i, e in self.m.items() if e.bad and i in self.stepped]
+
+    def remove(self, i):
+        del self.m[i]
+
+    def get_state_f64(self, i):
+        return None
+
+    def close(self):
+        pass
+
+
+class _Clock:
+    """A clock that advances only when the loop sleeps or a block runs (1 ms per step)."""
+
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 0.0)
+
+
+def _fake_serve(monkeypatch, reqs, **kw):
+    clk = _Clock()
+    batches = []
+
+    def make(graph_steps=0):
+        b = _FakeBatch(graph_steps)
+        orig = b.step
+
+        def step(k):
+            orig(k)
+            clk.t += 1e-3 * k
+        b.step = step
+        batches.append(b)
+        return b
+    monkeypatch.setattr(serving, "Batch", make)
+    monkeypatch.setattr(serving, "estimate_device_bytes", lambda s, bnd: s.nx * s.ny)
+    out = serving.serve(reqs, clock=clk, sleep=clk.sleep, **kw)
+    return out, batches[0]
+
+
+def test_serve_logic_budget_srpt_and_eviction_on_cpu(monkeypatch):
+    """The serving loop's host logic without a GPU: every request completes once, the HBM
+    budget bounds the resident cells, "srpt" completes requests in order of their work when one
+    runs at a time, and a request whose state is not finite is evicted as failed."""
+    reqs = S.serving_requests(8, 50.0, seed=9, sides=(16, 24, 32, 40), steps_range=(20, 200))
+    (recs, summ, _), b = _fake_serve(monkeypatch, reqs, block=16, hbm_budget=2 * 40 * 40)
+    assert summ["requests"] == 8 and summ["failed"] == 0
+    assert sorted(r.rid for r in recs) == list(range(8))
+    for rec in recs:
+        assert rec.finished >= rec.admitted >= rec.arrival - 1e-12
+    # at most the budget's worth of cells resident: never more than two 40^2 grids
+    assert all(len(run) <= 8 for _, run in b.log)
+    # srpt with room for one grid at a time: completion in order of work
+    at_once = [S.Request(r.rid, 0.0, r.scenario, r.steps) for r in reqs]
+    (recs, summ, _), b = _fake_serve(monkeypatch, at_once, block=16, policy="srpt", gpu_cells=1)
+    work = {r.rid: r.scenario.s.nx * r.scenario.s.ny * r.steps for r in reqs}
+    order = [rec.rid for rec in sorted(recs, key=lambda rec: rec.finished)]
+    assert [work[i] for i in order] == sorted(work.values())
+    assert all(len(run) == 1 for _, run in b.log)
+    # a request with a NaN in its state: evicted after its first block, the others finish
+    bad = S.small(nx=20, ny=20, seed=3)
+    bad.state[0, 5, 5] = np.nan
+    mixed = at_once[:3] + [S.Request(99, 0.0, bad, 500)]
+    (recs, summ, _), b = _fake_serve(monkeypatch, mixed, block=16)
+    assert summ["failed"] == 1 and summ["requests"] == 3
+    failed = [r for r in recs if r.failed]
+    assert failed[0].rid == 99 and failed[0].finished < max(r.finished for r in recs if not r.failed)
