@@ -85,6 +85,7 @@ ShallowWaterEngine::ShallowWaterEngine(const cel_settings &s, const cel_boundary
     current_timestep_ = old_timestep_ = old_old_timestep_ = s.timestep;
     uniform_dt_ = s.timestep;
     check(hipGetDevice(&device_), "hipGetDevice");
+    rows_ = default_band_rows(device_);
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
     allocate();
@@ -1156,6 +1157,22 @@ void ShallowWaterEngine::resetTimestepAdaptive(const cel_adaptive &a, cel_stats 
 }
 
 // ---- work decomposition
+
+// The tallest of 12-, 8- and 4-row bands whose one-wave tiles still number at least the
+// device's resident waves (4 per SIMD): taller bands re-read fewer halo rows, but a grid
+// whose bands do not fill the device is latency-bound and runs faster in more, shorter
+// bands (measured on MI355X: 4 rows up to 1024^2, 8 at 1536^2, 12 at 2048^2).
+int ShallowWaterEngine::default_band_rows(int device) const
+{
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return kDefaultBandRows;
+    const long long slots = 16LL * cus;
+    const long long strips = (nx_ + kStripCols - 1) / kStripCols;
+    for (int r : {12, 8})
+        if (strips * ((ny_ + r - 1) / r) >= slots) return r;
+    return 4;
+}
 
 void ShallowWaterEngine::getTuning(cel_tuning *out) const
 {

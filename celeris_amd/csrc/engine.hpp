@@ -164,6 +164,7 @@ private:
     void freeSideTables(int side);
     void validateBoundaries() const;
     void fillExplicit(ExplicitParams &p) const;
+    int default_band_rows(int device) const;
     void fillLines(LineParams &p) const;
     void fillStatic(StaticParams &p) const;
     void check(hipError_t e, const char *what) const;
