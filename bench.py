@@ -297,16 +297,16 @@ def ensemble_line(args):
     }
 
 
-def precision_line(args):
-    """The precision case at its full size: 4096 x 1024 channel (dx 0.25 m, 1:50 beach to
-    +0.5 m, solitary wave H/h = 0.3, walls), run in FP32, compensated FP32 and FP64 on the GPU.
-    The FP64 engine is the reference (it equals the FP64 CPU oracle to round-off at sizes the
-    oracle can run); each variant reports its eta error over H at the end, its maximum
-    run-up on the centre row, its volume drift and its throughput."""
+def precision_variants(nx, ny, steps):
+    """The precision case: nx x ny channel (dx 0.25 m, 1:50 beach to +0.5 m, solitary wave
+    H/h = 0.3, walls), run in FP32, compensated FP32 and FP64 on the GPU.  The FP64 engine is
+    the reference (it equals the FP64 CPU oracle to round-off at sizes the oracle can run);
+    each variant reports its eta error over H at the end, its maximum run-up on the centre
+    row, its volume drift and its throughput."""
     from celeris_amd import STATE_COMPENSATED, STATE_FP32, STATE_FP64, ShallowWaterEngine
     from celeris_amd import scenarios as S
     H, chunk = 0.3, 2000
-    sc = S.solitary_channel(nx=args.nx, ny=args.ny, dx=0.25, steps=args.steps)
+    sc = S.solitary_channel(nx=nx, ny=ny, dx=0.25, steps=steps)
     row = sc.s.ny // 2 + 2
     cells = sc.s.nx * sc.s.ny
     xy = [(i, row) for i in range(sc.s.nx + 4)]
@@ -317,8 +317,8 @@ def precision_line(args):
         e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
         v0 = e.stats()["sum_h"]
         runup, wall, done = -1e9, 0.0, 0
-        while done < args.steps:
-            k = min(chunk, args.steps - done)
+        while done < steps:
+            k = min(chunk, steps - done)
             t0 = time.perf_counter()
             e.timestep(k)
             e.sync()
@@ -328,9 +328,9 @@ def precision_line(args):
             wet = (w - ba) > 1e-3
             if wet.any():
                 runup = max(runup, float(ba[wet].max()))
-            print(f"bench: precision {name} {done}/{args.steps}", file=sys.stderr, flush=True)
+            print(f"bench: precision {name} {done}/{steps}", file=sys.stderr, flush=True)
         finals[name] = e.get_state_f64()
-        out[name] = {"gcell_steps_per_s": cells * args.steps / wall / 1e9, "runup_m": runup,
+        out[name] = {"gcell_steps_per_s": cells * steps / wall / 1e9, "runup_m": runup,
                      "volume_drift": (e.stats()["sum_h"] - v0) / v0}
         e.close()
     ref = finals["fp64"][0][2:-2, 2:-2]
@@ -338,11 +338,49 @@ def precision_line(args):
         d = finals[name][0][2:-2, 2:-2] - ref
         out[name]["eta_linf_over_H"] = float(np.abs(d).max() / H)
         out[name]["eta_l2_over_H"] = float(np.sqrt((d ** 2).mean()) / H)
+    workload = (f"solitary wave H/h 0.3 on a 1:50 beach, {sc.s.nx}x{sc.s.ny} cells, dx 0.25 m, "
+                f"dt {sc.s.timestep:.5f} s, walls, {steps} steps")
+    return workload, out
+
+
+def precision_line(args):
+    """The precision case at its full size and length (4096 x 1024, 100 000 steps)."""
+    workload, out = precision_variants(args.nx, args.ny, args.steps)
     return {"metric": "precision case (eta error vs GPU FP64)", "value": out["compensated"]["eta_linf_over_H"],
             "unit": "L-inf eta error / H (compensated)", "n_gpus": 1, "steps": args.steps, "warmup": 0,
             "higher_is_better": False, "vs_baseline": None, "dtype": "f32 / f32+f32 / f64", "data": "synthetic",
-            "config": {"workload": f"solitary wave H/h 0.3 on a 1:50 beach, {sc.s.nx}x{sc.s.ny} cells, dx 0.25 m, "
-                                   f"dt {sc.s.timestep:.5f} s, walls", "variants": out}}
+            "config": {"workload": workload, "variants": out}}
+
+
+def precision_leg(steps=10000):
+    """The precision case at its full size for a bounded number of steps: the three GPU
+    variants against the GPU FP64 engine."""
+    workload, out = precision_variants(4096, 1024, steps)
+    return {"workload": workload, "variants": out}
+
+
+def precision_oracle_check(oracle_steps=20):
+    """CPU baseline of the precision case: the GPU FP64 engine against the CPU FP64 oracle
+    (OpenMP) over oracle_steps steps of the 4096 x 1024 channel, and the oracle's rate."""
+    from celeris_amd import STATE_FP64, ShallowWaterEngine
+    from celeris_amd import scenarios as S
+    from oracle.binding import Oracle, get_threads
+    sc = S.solitary_channel(nx=4096, ny=1024, dx=0.25, steps=oracle_steps)
+    sc.s.state_precision = STATE_FP64
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.timestep(oracle_steps)
+    a = e.get_state_f64()
+    e.close()
+    o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f64")
+    t0 = time.perf_counter()
+    o.step(oracle_steps)
+    cpu_s = time.perf_counter() - t0
+    b = o.state().astype(np.float64)
+    scale = [np.abs(b[0]).max(), max(np.abs(b[1]).max(), np.abs(b[2]).max())]
+    err = {"eta": float(np.abs(a[0] - b[0]).max() / scale[0]), "hu": float(np.abs(a[1] - b[1]).max() / scale[1]),
+           "hv": float(np.abs(a[2] - b[2]).max() / scale[1])}
+    return {"steps": oracle_steps, "relative_error": err,
+            "cpu_oracle_gcell_steps_per_s": 4096 * 1024 * oracle_steps / cpu_s / 1e9, "cpu_threads": get_threads()}
 
 
 def main():
@@ -360,6 +398,8 @@ def main():
     ap.add_argument("--band-rows", type=int, default=0, choices=[0, 4, 8, 12, 16],
                     help="pin the explicit kernel's band height after autotuning (0: keep the autotuned one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-steps", type=int, default=10000,
+                    help="steps of the 4096x1024 precision case in the default line (0: skip)")
     ap.add_argument("--precision", choices=list(PRECISIONS), default="fp32",
                     help="state precision of the headline workload (the headline figure is fp32)")
     ap.add_argument("--workload", choices=["headline", "ensemble", "precision", "cpu", "bandwidth", "serving"], default="headline")
@@ -498,7 +538,16 @@ def main():
         out["ensemble"] = ensemble_leg(args.ensemble_members)
         print(f"bench: ensemble {out['ensemble']['batched_gcell_steps_per_s']:.2f} Gcell-steps/s batched, "
               f"{out['ensemble']['speedup']:.2f}x one at a time", file=sys.stderr, flush=True)
+    if args.precision_steps > 0:
+        out["precision"] = precision_leg(args.precision_steps)
+        v = out["precision"]["variants"]
+        print(f"bench: precision eta Linf/H fp32 {v['fp32']['eta_linf_over_H']:.2e}, compensated "
+              f"{v['compensated']['eta_linf_over_H']:.2e}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
+        if args.precision_steps > 0:
+            out["precision"]["fp64_vs_cpu_fp64_oracle"] = precision_oracle_check()
+            print(f"bench: precision FP64 vs CPU FP64 oracle {out['precision']['fp64_vs_cpu_fp64_oracle']['relative_error']}",
+                  file=sys.stderr, flush=True)
         out["plain_256"] = plain_leg()
         print(f"bench: plain 256^2 {out['plain_256']['gpu_gcell_steps_per_s']:.2f} Gcell-steps/s", file=sys.stderr,
               flush=True)
