@@ -758,7 +758,7 @@ __device__ __forceinline__ void irregular_line(const IrregularSide &S, int strip
 // 2^k <= C they lie inside one lane's chunk, above that at the chunk's last slot of
 // lanes 2^(k-log2 C) (m+1) - 1.  Neighbours outside the chunk come from other lanes
 // through cross-lane reads; positions >= n are the reference's (0, 1, 0 | 0) margin.
-struct Eq { real a, b, c, d; };
+struct alignas(16) Eq { real a, b, c, d; };
 
 __device__ __forceinline__ Eq shfl_eq(const Eq &e, int src)
 {
@@ -894,7 +894,8 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
         // the chunk's last level-k unknown needs the next thread's level-(k-1) unknown at s-1
         sm.xch[k & 1][t] = q[s - 1];
         __syncthreads();
-        Eq nr = t + 1 < NT ? sm.xch[k & 1][t + 1] : margin;
+        Eq nr = sm.xch[k & 1][t + 1 < NT ? t + 1 : t];   // one 16-byte read, then the margin
+        if (t + 1 >= NT) nr = margin;
         if (base + C - 1 + s >= n) nr = margin;
 #pragma unroll
         for (int j = 0; j < (C >> k); ++j) {
