@@ -372,6 +372,18 @@ void ShallowWaterEngine::restore(const void *buf, size_t bytes)
     double scal[nscal];
     std::memcpy(scal, q, sizeof(scal));
     q += sizeof(scal);
+    // the scalars index host arrays (buffer and ring slots, time-series rows): a damaged
+    // checkpoint is rejected before anything is changed
+    auto whole = [](double v, double lo, double hi) { return v == std::floor(v) && v >= lo && v <= hi; };
+    const double imax = 2147483647.0;
+    bool ok = whole(scal[0], 0, 1) && whole(scal[1], 0, 2) && whole(scal[2], 0, 2) && whole(scal[3], 0, 2) &&
+              scal[1] != scal[2] && scal[1] != scal[3] && scal[2] != scal[3] && whole(scal[4], CEL_EULER, 3) &&
+              whole(scal[5], 0, imax) && whole(scal[6], 0, imax) && std::isfinite(scal[7]) &&
+              whole(scal[16], 0, 1) && whole(scal[17], 0, 1) && std::isfinite(scal[18]) && scal[15] > 0.0 &&
+              std::isfinite(scal[15]);
+    for (int k = 8; k <= 10; ++k) ok = ok && scal[k] > 0.0 && std::isfinite(scal[k]);
+    for (int k = 0; k < 4; ++k) ok = ok && whole(scal[11 + k], 1, imax);
+    if (!ok) throw Error(CEL_EINVAL, "checkpoint holds an invalid step state");
     sync();
     clearGraphs();
     for (const auto &pl : planes) {

@@ -92,3 +92,27 @@ def test_restore_rejects_another_engine():
     with pytest.raises(CeleriError):
         b.restore(ck[:100])
     b.close()
+
+
+def test_restore_rejects_a_damaged_step_state():
+    """A checkpoint whose step scalars are out of range (a buffer or ring slot, the scheme,
+    a time-series row before the first) is refused before anything changes: the engine
+    continues as if the restore had not been tried."""
+    sc = every_boundary(56)
+    a = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    a.timestep(5)
+    ck = a.checkpoint()
+    header = 12 * 4                      # magic + eleven ints
+    scal = ck[header:header + 19 * 8].view(np.float64)
+    for k, value in [(0, 2.0), (1, 3.0), (2, 0.5), (1, float(scal[2])), (4, 7.0), (8, float("nan")), (11, 0.0)]:
+        bad = ck.copy()
+        bad[header:header + 19 * 8].view(np.float64)[k] = value
+        with pytest.raises(CeleriError) as e:
+            a.restore(bad)
+        assert e.value.status == 1
+    a.timestep(7)
+    b = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    b.timestep(12)
+    assert np.array_equal(a.get_state_f64(), b.get_state_f64()) and a.time == b.time
+    a.close()
+    b.close()
