@@ -175,6 +175,27 @@ def test_lines_longer_than_a_wave_chunk():
     check(sc, a, b, exact=True)
 
 
+def test_longest_lines_and_smallest_grids():
+    """Lines at the library's maximum (CEL_MAX_LINE = 8192 unknowns: the (16, 8) block shape,
+    all 512 threads' chunks full) in either direction, a 4097-cell line (one past the (16, 4)
+    shape's capacity), and the smallest grids (4 x 4 and 4 x 5 interior cells: one strip, one
+    band, every interior cell next to a wall)."""
+    for nx, ny in ((8192, 6), (6, 8192), (4097, 5), (4, 4), (4, 5)):
+        sc = S.small(nx=nx, ny=ny, seed=15)
+        a, b = run_pair(sc, 20)
+        check(sc, a, b, exact=True)
+
+
+def test_lines_above_the_maximum_rejected():
+    from celeris_amd import CeleriError
+    for nx, ny in ((8193, 8), (8, 8193), (3, 8)):
+        sc = S.small(nx=max(nx, 4), ny=max(ny, 4), seed=16)
+        sc.s.nx, sc.s.ny = nx, ny
+        with pytest.raises(CeleriError) as e:
+            ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+        assert e.value.status == 1
+
+
 def test_level0_coefficients_match_oracle():
     """A, B, C of both line directions, ghost rows/columns and margins included."""
     for kw in (dict(nx=37, ny=29, seed=3), dict(nx=40, ny=30, seed=4, is_boussinesq=0), dict(nx=33, ny=41, seed=5,
