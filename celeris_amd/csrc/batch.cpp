@@ -424,7 +424,10 @@ int Batch::autotune(int repeats, bool tune_lines)
 
 // The line solves run one launch per block-shape class.  Each member's own best shapes can
 // leave several short launches whose tails idle the GPU, so the same lines under one shape
-// shared by every member (a single launch) are timed too, and the faster plan is kept.
+// shared by every member (a single launch) are timed too, and so are the chunk-size families:
+// every line direction takes the fewest waves of one chunk size (4, 8 or 16 unknowns per
+// thread) whose block holds it, so short lines keep small blocks in at most four launches.
+// The faster plan is kept.
 void Batch::tuneSharedLineShape(int repeats)
 {
     std::vector<ShallowWaterEngine *> act;
@@ -462,24 +465,39 @@ void Batch::tuneSharedLineShape(int repeats)
         check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event");
         return ms;
     };
-    std::vector<int> shapes = {-1};            // -1: every member's own shapes
+    // the smallest shape of chunk size C that holds n unknowns (-1: none)
+    auto family_shape = [](int C, int n) {
+        int best = -1;
+        for (int sh = 0; sh < kNumShapes; ++sh)
+            if (kShapeC[sh] == C && shape_capacity(sh) >= n && (best < 0 || kShapeW[sh] < kShapeW[best])) best = sh;
+        return best;
+    };
+    // candidates: {shared shape or -1, chunk-size family or 0}; {-1, 0} is every member's own shapes
+    std::vector<std::pair<int, int>> cands = {{-1, 0}};
+    for (int C : {4, 8, 16})
+        if (family_shape(C, nmax) >= 0) cands.emplace_back(-1, C);
     for (int sh = 0; sh < kNumShapes; ++sh)
-        if (shape_capacity(sh) >= nmax && shape_capacity(sh) <= 4 * nmax) shapes.push_back(sh);
-    std::vector<float> ms(shapes.size(), -1.f);
+        if (shape_capacity(sh) >= nmax && shape_capacity(sh) <= 4 * nmax) cands.emplace_back(sh, 0);
+    auto apply = [&](const std::pair<int, int> &c) {
+        for (size_t i = 0; i < act.size(); ++i) {
+            if (c.first >= 0) act[i]->setLineShapes(c.first, c.first);
+            else if (c.second > 0)
+                act[i]->setLineShapes(family_shape(c.second, act[i]->lineLengthX()),
+                                      family_shape(c.second, act[i]->lineLengthY()));
+            else act[i]->setLineShapes(own[i].first, own[i].second);
+        }
+    };
+    std::vector<float> ms(cands.size(), -1.f);
     for (int round = 0; round < 3; ++round)
-        for (size_t c = 0; c < shapes.size(); ++c) {
-            for (size_t i = 0; i < act.size(); ++i)
-                if (shapes[c] < 0) act[i]->setLineShapes(own[i].first, own[i].second);
-                else act[i]->setLineShapes(shapes[c], shapes[c]);
+        for (size_t c = 0; c < cands.size(); ++c) {
+            apply(cands[c]);
             const float t = time_solves();
             if (ms[c] < 0.f || t < ms[c]) ms[c] = t;
         }
     size_t best = 0;
-    for (size_t c = 1; c < shapes.size(); ++c)
+    for (size_t c = 1; c < cands.size(); ++c)
         if (ms[c] < ms[best]) best = c;
-    for (size_t i = 0; i < act.size(); ++i)
-        if (shapes[best] < 0) act[i]->setLineShapes(own[i].first, own[i].second);
-        else act[i]->setLineShapes(shapes[best], shapes[best]);
+    apply(cands[best]);
 }
 
 void Batch::step(int nsteps)

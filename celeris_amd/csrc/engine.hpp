@@ -124,6 +124,8 @@ public:
     void setTuning(const cel_tuning &t) override;
     // line-solve block shapes (indices into kShapeC / kShapeW) as a batch chooses them
     int lineLength() const { return nx_ > ny_ ? nx_ : ny_; }     // the longer line direction
+    int lineLengthX() const { return nx_; }
+    int lineLengthY() const { return ny_; }
     int shapeX() const { return shape_x_; }
     int shapeY() const { return shape_y_; }
     void setLineShapes(int sx, int sy);
