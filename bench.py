@@ -163,6 +163,13 @@ def device_bandwidth(nbytes=1 << 30, launches=20):
     return {k: round(probe_bandwidth(k, nbytes, launches)[1], 1) for k in ("read", "write", "copy")}
 
 
+def stencil_pattern(nx, ny, band_rows, launches=20):
+    """The explicit stencil's memory-access pattern without its arithmetic (the library's probe)
+    in the stencil's layout and two others: microseconds per launch."""
+    from celeris_amd import PATTERN_LAYOUTS, probe_stencil_pattern
+    return {f"{k}_us": round(1000 * probe_stencil_pattern(k, nx, ny, band_rows, launches), 2) for k in PATTERN_LAYOUTS}
+
+
 def bandwidth_line():
     from celeris_amd import probe_bandwidth
     rows = {}
@@ -530,6 +537,12 @@ def main():
     print(f"bench: {sc.s.nx}x{sc.s.ny} {args.precision} step {value:.2f} Gcell-steps/s", file=sys.stderr, flush=True)
     out["device_bandwidth_gb_per_s"] = device_bandwidth()
     print(f"bench: probes {out['device_bandwidth_gb_per_s']}", file=sys.stderr, flush=True)
+    if args.precision == "fp32" and tuning["band_rows"] in (8, 12, 16):
+        # the stencil's data movement alone, same grid and band height, beside its measured time
+        pat = stencil_pattern(sc.s.nx, sc.s.ny, tuning["band_rows"])
+        pat["explicit_step_us"] = round(1000 * avg_ms, 2)
+        out["roofline"]["access_pattern_probe"] = pat
+        print(f"bench: stencil access pattern alone {pat}", file=sys.stderr, flush=True)
     if args.serving_requests > 0:
         rates = [float(v) for v in args.serving_rates.split(",")]
         out["serving"] = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)

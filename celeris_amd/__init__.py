@@ -125,6 +125,8 @@ SIGNATURES = {
     "cel_engine_reset_timestep": ([ctypes.c_void_p, ctypes.POINTER(CelAdaptive), ctypes.POINTER(CelStats)],
                                   ctypes.c_int),
     "cel_probe_bandwidth": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _F], ctypes.c_int),
+    "cel_probe_stencil_pattern": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F],
+                                  ctypes.c_int),
     "cel_engine_check_numeric": ([ctypes.c_void_p], ctypes.c_int),
     "cel_batch_check_numeric": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -237,6 +239,18 @@ def probe_bandwidth(kind, nbytes=1 << 30, launches=20):
     _check(load_library().cel_probe_bandwidth(PROBE_KINDS[kind], nbytes, launches, ctypes.byref(ms)))
     moved = nbytes * (2 if kind == "copy" else 1)
     return ms.value, moved / (ms.value * 1e-3) / 1e9
+
+
+PATTERN_LAYOUTS = {"stencil": 0, "row_major_hv_dy": 1, "aligned_64": 2}
+
+
+def probe_stencil_pattern(layout="stencil", nx=2048, ny=2048, band_rows=12, launches=20):
+    """Average launch time (ms) of the explicit stencil's memory-access pattern without its
+    arithmetic (cel_probe_stencil_pattern) in one of PATTERN_LAYOUTS."""
+    ms = ctypes.c_float()
+    _check(load_library().cel_probe_stencil_pattern(nx, ny, band_rows, PATTERN_LAYOUTS[layout], launches,
+                                                    ctypes.byref(ms)))
+    return ms.value
 
 
 class ShallowWaterEngine:

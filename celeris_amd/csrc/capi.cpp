@@ -16,6 +16,7 @@ void bottom_from_corners(int nx, int ny, const float *z, float *bottom3);
 void initial_state(const cel_settings &s, const float *bottom3, const cel_soliton *sol, int nsol, float *state3,
                    const float *base3);
 float probe_bandwidth(int kind, size_t bytes, int launches);
+float probe_stencil_pattern(int nx, int ny, int rows, int layout, int launches);
 hipError_t stats_noalloc(int nx, int ny, int pitch, const float *w, const float *hu, const float *hv, const float *BA,
                          float g, float one_over_dx, float one_over_dy, float epsilon, double *out9, hipStream_t st);
 }
@@ -477,6 +478,14 @@ int cel_probe_bandwidth(int kind, size_t bytes, int launches, float *ms)
 {
     return guarded([&] {
         const float t = cel::probe_bandwidth(kind, bytes, launches);
+        if (ms) *ms = t;
+    });
+}
+
+int cel_probe_stencil_pattern(int nx, int ny, int band_rows, int layout, int launches, float *ms)
+{
+    return guarded([&] {
+        const float t = cel::probe_stencil_pattern(nx, ny, band_rows, layout, launches);
         if (ms) *ms = t;
     });
 }

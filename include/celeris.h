@@ -395,6 +395,15 @@ int cel_engine_coefficients(cel_engine *e, float *abc_x3, float *abc_y3);
  * counterpart in the reference. */
 int cel_probe_bandwidth(int kind, size_t bytes, int launches, float *ms);
 
+/* The explicit stencil's memory-access pattern without its arithmetic, on an nx x ny grid
+ * (4..8192 per side) with band_rows-row bands (8, 12, 16): the same tiles, row loads and
+ * stores, transposed hv staging and D_y stores.  layout 0: the stencil's (60-column strips);
+ * 1: the same with hv and D_y row-major (no transposed accesses); 2: 64-column strips
+ * aligned to 128-byte lines.  `launches` timed launches after one warm-up, *ms = average
+ * launch time: how long the stencil's data movement alone takes.  Allocates its own planes.
+ * No counterpart in the reference. */
+int cel_probe_stencil_pattern(int nx, int ny, int band_rows, int layout, int launches, float *ms);
+
 #ifdef __cplusplus
 }
 #endif
