@@ -170,6 +170,15 @@ def stencil_pattern(nx, ny, band_rows, launches=20):
     return {f"{k}_us": round(1000 * probe_stencil_pattern(k, nx, ny, band_rows, launches), 2) for k in PATTERN_LAYOUTS}
 
 
+def solve_pattern(lines, launches=20):
+    """The line solves' structure without their coefficient arithmetic (the library's probe):
+    microseconds per launch over `lines` x- and y-lines of 2048 unknowns, equations combined by
+    additions only and by cyclic reduction's eliminations."""
+    from celeris_amd import probe_solve_pattern
+    return {"additions_us": round(1000 * probe_solve_pattern(lines, False, launches), 2),
+            "eliminations_us": round(1000 * probe_solve_pattern(lines, True, launches), 2)}
+
+
 def bandwidth_line():
     from celeris_amd import probe_bandwidth
     rows = {}
@@ -543,6 +552,12 @@ def main():
         pat["explicit_step_us"] = round(1000 * avg_ms, 2)
         out["roofline"]["access_pattern_probe"] = pat
         print(f"bench: stencil access pattern alone {pat}", file=sys.stderr, flush=True)
+    if args.precision == "fp32" and sc.s.nx == 2048 and sc.s.ny == 2048:
+        # the line solves' loads, exchanges, barriers and stores alone, beside their measured time
+        sp = solve_pattern(2048)
+        sp["solve_lines_us"] = round(1000 * kern["solve_lines"], 2)
+        out["line_solve_probe"] = sp
+        print(f"bench: line-solve structure alone {sp}", file=sys.stderr, flush=True)
     if args.serving_requests > 0:
         rates = [float(v) for v in args.serving_rates.split(",")]
         out["serving"] = serving_leg(args.serving_requests, rates, 5, args.graph_steps or 50)

@@ -127,6 +127,7 @@ SIGNATURES = {
     "cel_probe_bandwidth": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _F], ctypes.c_int),
     "cel_probe_stencil_pattern": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F],
                                   ctypes.c_int),
+    "cel_probe_solve_pattern": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _F], ctypes.c_int),
     "cel_engine_check_numeric": ([ctypes.c_void_p], ctypes.c_int),
     "cel_batch_check_numeric": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -250,6 +251,15 @@ def probe_stencil_pattern(layout="stencil", nx=2048, ny=2048, band_rows=12, laun
     ms = ctypes.c_float()
     _check(load_library().cel_probe_stencil_pattern(nx, ny, band_rows, PATTERN_LAYOUTS[layout], launches,
                                                     ctypes.byref(ms)))
+    return ms.value
+
+
+def probe_solve_pattern(lines=2048, eliminate=True, launches=20):
+    """Average launch time (ms) of the line solves' structure without their coefficient
+    arithmetic (cel_probe_solve_pattern): `lines` x-lines and as many y-lines of 2048 unknowns,
+    equations combined by cyclic reduction's eliminations or by additions only."""
+    ms = ctypes.c_float()
+    _check(load_library().cel_probe_solve_pattern(lines, 1 if eliminate else 0, launches, ctypes.byref(ms)))
     return ms.value
 
 

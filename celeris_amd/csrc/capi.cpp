@@ -17,6 +17,7 @@ void initial_state(const cel_settings &s, const float *bottom3, const cel_solito
                    const float *base3);
 float probe_bandwidth(int kind, size_t bytes, int launches);
 float probe_stencil_pattern(int nx, int ny, int rows, int layout, int launches);
+float probe_solve_pattern(int lines, int elim, int launches);
 hipError_t stats_noalloc(int nx, int ny, int pitch, const float *w, const float *hu, const float *hv, const float *BA,
                          float g, float one_over_dx, float one_over_dy, float epsilon, double *out9, hipStream_t st);
 }
@@ -486,6 +487,14 @@ int cel_probe_stencil_pattern(int nx, int ny, int band_rows, int layout, int lau
 {
     return guarded([&] {
         const float t = cel::probe_stencil_pattern(nx, ny, band_rows, layout, launches);
+        if (ms) *ms = t;
+    });
+}
+
+int cel_probe_solve_pattern(int lines, int elim, int launches, float *ms)
+{
+    return guarded([&] {
+        const float t = cel::probe_solve_pattern(lines, elim, launches);
         if (ms) *ms = t;
     });
 }

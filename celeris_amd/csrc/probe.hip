@@ -145,6 +145,104 @@ __global__ __launch_bounds__(64, 4) void probe_pattern(PatternPlanes p)
     }
 }
 
+// ---- the line solves' structure, without their coefficient arithmetic
+//
+// One 256-thread block per line of 2048 unknowns (8 per thread, as the (8, 4) shape of
+// solve_lines_block), lines of both directions in one launch: 16-byte loads of the right-hand
+// side and the bed, the three chunk levels (one LDS exchange and barrier each), eight
+// cross-thread reduction and nine substitution levels in LDS (a barrier each), the chunk
+// substitution and 8-byte stores.  ELIM 0 combines equations with additions only; 1 with the
+// reference's eliminations (two quotients per reduced equation, one per substituted unknown)
+// on a diagonally dominant system.
+struct SolvePatternPlanes {
+    const float *D[2], *B[2];
+    float *out[2];
+    int lines, pitch;
+};
+
+struct alignas(16) PEq { float a, b, c, d; };
+
+template <int ELIM>
+__device__ __forceinline__ PEq probe_reduce(const PEq &L, const PEq &H, const PEq &R)
+{
+    if (ELIM == 0) return PEq{L.a + H.a, H.b + R.b, R.c + H.c, H.d + L.d + R.d};
+    const float k1 = H.a / L.b, k2 = H.c / R.b;
+    return PEq{-L.a * k1, H.b - L.c * k1 - R.a * k2, -R.c * k2, H.d - L.d * k1 - R.d * k2};
+}
+
+template <int ELIM>
+__global__ __launch_bounds__(256) void probe_solve(SolvePatternPlanes p)
+{
+    constexpr int C = 8, NT = 256;
+    __shared__ float ta[NT + 8], tb[NT + 8], tc[NT + 8], td[NT + 8];
+    __shared__ PEq xch[2][NT];
+    const int dir = blockIdx.x >= p.lines ? 1 : 0;
+    const int line = (int)blockIdx.x - dir * p.lines + 2;
+    const int t = threadIdx.x, base = t * C;
+    const float *D = p.D[dir] + (size_t)line * p.pitch, *B = p.B[dir] + (size_t)line * p.pitch;
+    float *out = p.out[dir] + (size_t)line * p.pitch + 2;
+    PEq q[C];
+    {
+        const float4 d0 = *reinterpret_cast<const float4 *>(D + base), d1 = *reinterpret_cast<const float4 *>(D + base + 4);
+        const float4 b0 = *reinterpret_cast<const float4 *>(B + base), b1 = *reinterpret_cast<const float4 *>(B + base + 4),
+                     b2 = *reinterpret_cast<const float4 *>(B + base + 8);
+        const float dv[C] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+        const float bv[C + 4] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+        for (int i = 0; i < C; ++i) q[i] = PEq{-bv[i], 1.f + bv[i + 2], -bv[i + 4], dv[i]};
+    }
+#pragma unroll
+    for (int k = 1; k <= 3; ++k) {
+        const int s = 1 << (k - 1);
+        xch[k & 1][t] = q[s - 1];
+        __syncthreads();
+        const PEq nr = xch[k & 1][t + 1 < NT ? t + 1 : t];
+#pragma unroll
+        for (int j = 0; j < (C >> k); ++j) {
+            const int pp = ((j + 1) << k) - 1;
+            q[pp] = probe_reduce<ELIM>(q[pp - s], q[pp], pp + s < C ? q[pp + s] : nr);
+        }
+    }
+    auto at = [](int i) { return i + (i >> 5); };
+    ta[at(t)] = q[C - 1].a; tb[at(t)] = q[C - 1].b; tc[at(t)] = q[C - 1].c; td[at(t)] = q[C - 1].d;
+    __syncthreads();
+#pragma unroll
+    for (int j = 1; j <= 8; ++j) {
+        const int s = 1 << (j - 1), i = ((t + 1) << j) - 1;
+        if (i < NT) {
+            const int l = at(i - s), m = at(i), r = i + s < NT ? at(i + s) : m;
+            const PEq o = probe_reduce<ELIM>(PEq{ta[l], tb[l], tc[l], td[l]}, PEq{ta[m], tb[m], tc[m], td[m]},
+                                             PEq{ta[r], tb[r], tc[r], td[r]});
+            ta[m] = o.a; tb[m] = o.b; tc[m] = o.c; td[m] = o.d;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 8; j >= 0; --j) {
+        const int s = 1 << j, i = ((2 * t + 1) << j) - 1;
+        if (i < NT) {
+            const float xl = i - s >= 0 ? td[at(i - s)] : 0.f, xr = i + s < NT ? td[at(i + s)] : 0.f;
+            const int m = at(i);
+            td[m] = ELIM == 0 ? td[m] + ta[m] * xl + tc[m] * xr : (td[m] - ta[m] * xl - tc[m] * xr) / tb[m];
+        }
+        __syncthreads();
+    }
+    q[C - 1].d = td[at(t)];
+    const float prev = t > 0 ? td[at(t - 1)] : 0.f;
+#pragma unroll
+    for (int k = 2; k >= 0; --k) {
+        const int s = 1 << k;
+#pragma unroll
+        for (int qq = 0; qq < (C >> (k + 1)); ++qq) {
+            const int pp = s * (2 * qq + 1) - 1;
+            const float xl = pp - s >= 0 ? q[pp - s].d : prev, xr = q[pp + s].d;
+            q[pp].d = ELIM == 0 ? q[pp].d + q[pp].a * xl + q[pp].c * xr : (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < C; i += 2) *reinterpret_cast<float2 *>(out + base + i) = make_float2(q[i].d, q[i + 1].d);
+}
+
 void check(hipError_t e, const char *what)
 {
     if (e != hipSuccess)
@@ -210,6 +308,66 @@ float probe_stencil_pattern(int nx, int ny, int rows, int layout, int launches)
             else CEL_PP(16);
 #undef CEL_PP
             check(hipGetLastError(), "pattern probe launch");
+        };
+        launch();
+        check(hipEventRecord(e0, st), "event");
+        for (int i = 0; i < launches; ++i) launch();
+        check(hipEventRecord(e1, st), "event");
+        check(hipEventSynchronize(e1), "event sync");
+        check(hipEventElapsedTime(&ms, e0, e1), "event");
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return ms / (float)launches;
+}
+
+// The line solves' structure (probe_solve) over `lines` x-lines and as many y-lines of 2048
+// unknowns (lines <= 8192): `launches` timed launches after one warm-up; returns the average
+// launch time in ms.  elim 0: additions only; 1: the reference's eliminations.  Allocates its
+// own planes.
+float probe_solve_pattern(int lines, int elim, int launches)
+{
+    if (lines < 1 || lines > 8192) throw Error(CEL_EINVAL, "lines must lie in [1, 8192]");
+    if (elim < 0 || elim > 1) throw Error(CEL_EINVAL, "elim must be 0 or 1");
+    if (launches < 1 || launches > 10000) throw Error(CEL_EINVAL, "launches must lie in [1, 10000]");
+    SolvePatternPlanes q{};
+    q.lines = lines;
+    q.pitch = (2048 + 20 + 63) / 64 * 64;
+    const size_t plane = (size_t)q.pitch * (lines + 4) * sizeof(float);
+    std::vector<void *> bufs;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    auto release = [&]() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void *b : bufs) (void)hipFree(b);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    try {
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        // bed planes filled with 0.1 (b = 1.1, a = c = -0.1: diagonally dominant), the rest zero
+        const std::vector<float> fill(plane / sizeof(float), 0.1f);
+        for (int k = 0; k < 6; ++k) {
+            void *m = nullptr;
+            check(hipMalloc(&m, plane), "hipMalloc(solve probe)");
+            bufs.push_back(m);
+            if (k == 2 || k == 3) check(hipMemcpyAsync(m, fill.data(), plane, hipMemcpyHostToDevice, st), "upload");
+            else check(hipMemsetAsync(m, 0, plane, st), "hipMemset(solve probe)");
+        }
+        q.D[0] = static_cast<float *>(bufs[0]); q.D[1] = static_cast<float *>(bufs[1]);
+        q.B[0] = static_cast<float *>(bufs[2]); q.B[1] = static_cast<float *>(bufs[3]);
+        q.out[0] = static_cast<float *>(bufs[4]); q.out[1] = static_cast<float *>(bufs[5]);
+        check(hipStreamSynchronize(st), "sync");
+        auto launch = [&]() {
+            if (elim == 0) hipLaunchKernelGGL(probe_solve<0>, dim3(2 * lines), dim3(256), 0, st, q);
+            else hipLaunchKernelGGL(probe_solve<1>, dim3(2 * lines), dim3(256), 0, st, q);
+            check(hipGetLastError(), "solve probe launch");
         };
         launch();
         check(hipEventRecord(e0, st), "event");

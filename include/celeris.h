@@ -404,6 +404,15 @@ int cel_probe_bandwidth(int kind, size_t bytes, int launches, float *ms);
  * No counterpart in the reference. */
 int cel_probe_stencil_pattern(int nx, int ny, int band_rows, int layout, int launches, float *ms);
 
+/* The line solves' structure without their coefficient arithmetic: `lines` x-lines and as
+ * many y-lines (1..8192) of 2048 unknowns, one 256-thread block each, as the engine's line
+ * solve lays them out: 16-byte loads of the right-hand side and bed, three chunk levels with
+ * a neighbour exchange each, 17 cross-thread levels in LDS (a barrier each), 8-byte stores.
+ * elim 0 combines the equations with additions only; 1 with cyclic reduction's eliminations
+ * (diagonally dominant system).  `launches` timed launches after one warm-up, *ms = average
+ * launch time.  Allocates its own planes.  No counterpart in the reference. */
+int cel_probe_solve_pattern(int lines, int elim, int launches, float *ms);
+
 #ifdef __cplusplus
 }
 #endif
