@@ -115,8 +115,9 @@ void ShallowWaterEngine::allocate()
 {
     const size_t plane = (size_t)P_ * Y_ * sizeof(real);
     const size_t tplane = (size_t)PT_ * X_ * sizeof(real);
-    // w,hu x2; BY,BX,BA; Dx; G; Gp; FG; FGp; Cw,Chu; compensated w_lo, hu_lo x2
-    const int nrow = 2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + (comp_ ? 4 : 0);
+    // w,hu x2; BY,BX,BA; Dx; history ring (3 slots of a quad plane + a G* plane) and the
+    // predictor's slot; Cw,Chu; compensated w_lo, hu_lo x2
+    const int nrow = 2 + 2 + 3 + 1 + 3 * 5 + 5 + 2 + (comp_ ? 4 : 0);
     const int ncol = 2 + 1 + 1 + 1 + (comp_ ? 2 : 0);         // hvT x2; DyT; ChvT; BAT; hvT_lo x2
     arena_bytes_ = plane * nrow + tplane * ncol;
     void *mem = nullptr;
@@ -128,10 +129,9 @@ void ShallowWaterEngine::allocate()
     auto cp = [&]() { real *r = reinterpret_cast<real *>(q); q += tplane; return r; };
     w_[0] = rp(); w_[1] = rp(); hu_[0] = rp(); hu_[1] = rp();
     BY_ = rp(); BX_ = rp(); BA_ = rp(); Dx_ = rp();
-    for (auto &g : G_) for (auto &c : g) c = rp();
-    for (auto &c : Gp_) c = rp();
-    for (auto &g : FG_) for (auto &c : g) c = rp();
-    for (auto &c : FGp_) c = rp();
+    auto rp4 = [&]() { real *r = reinterpret_cast<real *>(q); q += 4 * plane; return r; };
+    for (int k = 0; k < 3; ++k) { H_[k] = rp4(); H1_[k] = rp(); }
+    Hp_ = rp4(); Hp1_ = rp();
     Cw_ = rp(); Chu_ = rp();
     hvT_[0] = cp(); hvT_[1] = cp(); DyT_ = cp(); ChvT_ = cp(); BAT_ = cp();
     if (comp_) {
@@ -157,7 +157,7 @@ size_t ShallowWaterEngine::estimateBytes(const cel_settings &s, const cel_bounda
     const size_t P = round_up((size_t)X + 16, 64), PT = round_up((size_t)Y + 16, 64);
     const bool comp = s.state_precision == CEL_STATE_COMPENSATED;
     const size_t plane = P * Y * sizeof(real), tplane = PT * X * sizeof(real);
-    size_t b = plane * (2 + 2 + 3 + 1 + 9 + 3 + 6 + 2 + 2 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 1 + (comp ? 2 : 0));
+    size_t b = plane * (2 + 2 + 3 + 1 + 3 * 5 + 5 + 2 + (comp ? 4 : 0)) + tplane * (2 + 1 + 1 + 1 + (comp ? 2 : 0));
     const int tiles = ((4 * (s.nx / 4) + kStatsTileX - 1) / kStatsTileX) * ((4 * (s.ny / 4) + kStatsTileY - 1) / kStatsTileY);
     b += ((size_t)tiles + 1) * kStatsValues * sizeof(double) + 256;
     for (int side = 0; side < 4; ++side) {
@@ -269,8 +269,10 @@ void ShallowWaterEngine::setState(const float *state3)
     check(hipMemcpyAsync(hu_[1], hu_[0], plane, hipMemcpyDeviceToDevice, stream_), "copy hu");
     check(hipMemcpyAsync(hvT_[1], hvT_[0], tplane0, hipMemcpyDeviceToDevice, stream_), "copy hv");
     // history textures start at zero (engine.cpp:3024-3034 creates them without data)
-    for (auto &g : G_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
-    for (auto &g : FG_) for (auto &c : g) check(hipMemsetAsync(c, 0, plane, stream_), "zero history");
+    for (int k = 0; k < 3; ++k) {
+        check(hipMemsetAsync(H_[k], 0, 4 * plane, stream_), "zero history");
+        check(hipMemsetAsync(H1_[k], 0, plane, stream_), "zero history");
+    }
     if (comp_) {
         const size_t tplane = (size_t)PT_ * X_ * e;
         for (int k = 0; k < 2; ++k) {
@@ -295,7 +297,7 @@ void ShallowWaterEngine::setState(const float *state3)
 
 namespace {
 constexpr unsigned kCheckpointMagic = 0x43454c4bu;   // "CELK"
-constexpr int kCheckpointVersion = 1;
+constexpr int kCheckpointVersion = 2;   // 2: history slots as a quad plane + a G* plane
 struct CheckpointHeader {
     unsigned magic;
     int version, real_bytes, comp, nx, ny, P, PT, nplanes, nscalars, correction_steps, aux;
@@ -311,11 +313,9 @@ std::vector<std::pair<real *, size_t>> ShallowWaterEngine::checkpointPlanes() co
         for (int k = 0; k < 2; ++k) {
             v.emplace_back(w_lo_[k], plane); v.emplace_back(hu_lo_[k], plane); v.emplace_back(hvT_lo_[k], tplane);
         }
-    for (auto &g : G_) for (auto *c : g) v.emplace_back(c, plane);
-    for (auto &g : FG_) for (auto *c : g) v.emplace_back(c, plane);
+    for (int k = 0; k < 3; ++k) { v.emplace_back(H_[k], 4 * plane); v.emplace_back(H1_[k], plane); }
     if (s_.correction_steps_num > 0) {
-        for (auto *c : Gp_) v.emplace_back(c, plane);
-        for (auto *c : FGp_) v.emplace_back(c, plane);
+        v.emplace_back(Hp_, 4 * plane); v.emplace_back(Hp1_, plane);
         v.emplace_back(Cw_, plane); v.emplace_back(Chu_, plane); v.emplace_back(ChvT_, tplane);
     }
     if (aux_on_) v.emplace_back(aux_mem_, 3 * plane);
@@ -706,8 +706,10 @@ void ShallowWaterEngine::fillExplicit(ExplicitParams &p) const
     const int c = cur_, n = 1 - cur_;
     p.w = w_[c]; p.hu = hu_[c]; p.hvT = hvT_[c];
     p.BY = BY_; p.BX = BX_; p.BA = BA_;
-    for (int k = 0; k < 3; ++k) { p.Go[k] = G_[ring_old_][k]; p.Goo[k] = G_[ring_oo_][k]; p.Gs[k] = G_[ring_scr_][k]; p.Gp[k] = Gp_[k]; }
-    for (int k = 0; k < 2; ++k) { p.FGo[k] = FG_[ring_old_][k]; p.FGoo[k] = FG_[ring_oo_][k]; p.FGs[k] = FG_[ring_scr_][k]; p.FGp[k] = FGp_[k]; }
+    p.Ho = H_[ring_old_]; p.Ho1 = H1_[ring_old_];
+    p.Hoo = H_[ring_oo_]; p.Hoo1 = H1_[ring_oo_];
+    p.Hs = H_[ring_scr_]; p.Hs1 = H1_[ring_scr_];
+    p.Hp = Hp_; p.Hp1 = Hp1_;
     p.Cw = Cw_; p.Chu = Chu_; p.ChvT = ChvT_;
     p.w_new = w_[n]; p.Dx = Dx_; p.DyT = DyT_;
     p.w_lo = w_lo_[c]; p.w_lo_new = w_lo_[n];
@@ -819,8 +821,8 @@ void ShallowWaterEngine::planStep(StepLaunch &L, bool advance)
     if ((scheme == CEL_PREDICTOR || scheme == CEL_PREDICTOR_ADAPTIVE) && s_.correction_steps_num > 0) {
         const size_t plane = (size_t)P_ * Y_ * sizeof(real), tplane = (size_t)PT_ * X_ * sizeof(real);
         auto add = [&](real *d, const real *src, size_t bytes) { L.copy[L.ncopy++] = StepLaunch::Copy{d, src, bytes}; };
-        for (int k = 0; k < 3; ++k) add(Gp_[k], G_[ring_scr_][k], plane);
-        for (int k = 0; k < 2; ++k) add(FGp_[k], FG_[ring_scr_][k], plane);
+        add(Hp_, H_[ring_scr_], 4 * plane);
+        add(Hp1_, H1_[ring_scr_], plane);
         add(Cw_, w_[cur_], plane);
         add(Chu_, hu_[cur_], plane);
         add(ChvT_, hvT_[cur_], tplane);

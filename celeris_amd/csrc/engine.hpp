@@ -208,9 +208,10 @@ private:
     real *BY_, *BX_, *BA_;
     real *BAT_;                         // BA transposed (the y-lines' coefficients)
     real *Dx_, *DyT_;
-    real *G_[3][3];           // ring slot, component
-    real *FG_[3][2];
-    real *Gp_[3], *FGp_[2];
+    // Adams-Bashforth ring: per slot a quad plane (dU/dt of w, hu, hv and F*, four reals per
+    // cell) and a G* plane; the predictor's copy of a slot in the same layout
+    real *H_[3], *H1_[3];
+    real *Hp_, *Hp1_;
     real *Cw_, *Chu_, *ChvT_;
     std::vector<real> hostBA_;
     std::vector<real> hostBottom_;      // BY, BX, BA planes, ghost-inclusive (terrain edits)

@@ -110,6 +110,8 @@ __device__ __forceinline__ real sqrt_k(real x)
 constexpr int kVec = 16 / (int)sizeof(real);
 struct alignas(16) Vec { real v[kVec]; };
 struct alignas(2 * sizeof(real)) Pair { real x, y; };
+// the four history values of one cell in a quad plane (one 16-byte access in FP32)
+struct alignas(4 * sizeof(real)) Quad { real v[4]; };
 
 // ------------------------------------------------------------------ helpers
 
@@ -208,6 +210,15 @@ __device__ __forceinline__ void st32(real *base, unsigned idx, real v)
 {
     *reinterpret_cast<real *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(real)) = v;
 }
+// quad-plane element idx (grids up to 8192 x 8192 keep the byte offset below 4 GiB in FP64)
+__device__ __forceinline__ Quad ldq(const real *base, unsigned idx)
+{
+    return *reinterpret_cast<const Quad *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(Quad));
+}
+__device__ __forceinline__ void stq(real *base, unsigned idx, const Quad &v)
+{
+    *reinterpret_cast<Quad *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(Quad)) = v;
+}
 
 struct Face { real h, u, v; };
 
@@ -297,10 +308,10 @@ __device__ __forceinline__ CellIn load_cell(const ExplicitParams &p, unsigned i)
 {
     CellIn c;
     if (SCHEME != 0) {
-        c.o0 = ld32(p.Go[0], i); c.o1 = ld32(p.Go[1], i); c.o2 = ld32(p.Go[2], i);
-        c.oo0 = ld32(p.Goo[0], i); c.oo1 = ld32(p.Goo[1], i); c.oo2 = ld32(p.Goo[2], i);
-        c.Fo = ld32(p.FGo[0], i); c.Go = ld32(p.FGo[1], i);
-        c.Foo = ld32(p.FGoo[0], i); c.Goo = ld32(p.FGoo[1], i);
+        const Quad a = ldq(p.Ho, i), b = ldq(p.Hoo, i);
+        c.o0 = a.v[0]; c.o1 = a.v[1]; c.o2 = a.v[2]; c.Fo = a.v[3];
+        c.oo0 = b.v[0]; c.oo1 = b.v[1]; c.oo2 = b.v[2]; c.Foo = b.v[3];
+        c.Go = ld32(p.Ho1, i); c.Goo = ld32(p.Hoo1, i);
     } else {
         c.o0 = c.o1 = c.o2 = c.oo0 = c.oo1 = c.oo2 = c.Fo = c.Go = c.Foo = c.Goo = 0.f;
     }
@@ -602,8 +613,9 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
                     const real chv = chvblk[y - (ys - 2)][lane];
                     const real chvd = chvblk[y - 1 - (ys - 2)][lane];
                     const real chvu = chvblk[y + 1 - (ys - 2)][lane];
-                    const real pr0 = ld32(p.Gp[0], i), pr1 = ld32(p.Gp[1], i), pr2 = ld32(p.Gp[2], i);
-                    const real Fp = ld32(p.FGp[0], i), Gp_ = ld32(p.FGp[1], i);
+                    const Quad pq = ldq(p.Hp, i);
+                    const real pr0 = pq.v[0], pr1 = pq.v[1], pr2 = pq.v[2], Fp = pq.v[3];
+                    const real Gp_ = ld32(p.Hp1, i);
                     wOut = cw + dt / 24.f * (9.f * dr + 19.f * pr0 - 5.f * o0 + 1.f * oo0);
                     huOut = Ax * chul + Bx * chu + Cx * chur + dt / 24.f * (9.f * dg + 19.f * pr1 - 5.f * o1 + 1.f * oo1) +
                             F_star - Fp;
@@ -624,11 +636,8 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
             st32(p.w_new, i, wOut);
             st32(p.Dx, i - 2u, huOut);
             dystg[y - ys][lane] = hvOut;
-            st32(p.Gs[0], i, dr);
-            st32(p.Gs[1], i, dg);
-            st32(p.Gs[2], i, db);
-            st32(p.FGs[0], i, F_star);
-            st32(p.FGs[1], i, G_star);
+            stq(p.Hs, i, Quad{{dr, dg, db, F_star}});
+            st32(p.Hs1, i, G_star);
         }
         // ---- shift windows
         w_[0] = w_[1]; w_[1] = w_[2]; w_[2] = w_[3]; w_[3] = w_[4];

@@ -51,15 +51,17 @@ struct ExplicitParams {
     real *w_lo_new;
     // bottom: BY (north face), BX (east face), BA (centre)
     const real *BY, *BX, *BA;
-    // Adams-Bashforth history: dU/dt at n-1 and n-2 (w, hu, hv planes), (F*, G*) at n-1, n-2
-    const real *Go[3], *Goo[3], *FGo[2], *FGoo[2];
-    // corrector inputs: predicted dU/dt and (F*, G*), and the state the predictor started from
-    const real *Gp[3], *FGp[2], *Cw, *Chu, *ChvT;
+    // Adams-Bashforth history, one ring slot per step: the dU/dt of (w, hu, hv) and F* of a
+    // cell as four consecutive reals (a quad plane, [4 (y P + x) + k]) and G* in a plane of
+    // its own.  Slots of steps n-1 and n-2:
+    const real *Ho, *Ho1, *Hoo, *Hoo1;
+    // corrector inputs: the predicted slot (same layout), and the state the predictor started from
+    const real *Hp, *Hp1, *Cw, *Chu, *ChvT;
     // outputs
     real *w_new;    // w^{n+1} (interior)
     real *Dx;       // right-hand side of the x-line systems, [y * P + (x - 2)]
     real *DyT;      // right-hand side of the y-line systems, [x * PT + (y - 2)]
-    real *Gs[3], *FGs[2];
+    real *Hs, *Hs1;   // history slot of step n (quad plane, G* plane)
     // numeric sentinel: bit 0 is raised when a new w or a line right-hand side is not finite
     // (null: not checked)
     int *status;

@@ -54,17 +54,20 @@ __global__ __launch_bounds__(kProbeThreads) void probe_copy(const float4 *__rest
 // ---- the explicit stencil's memory-access pattern, without its arithmetic
 //
 // One wave per (strip, band) tile as in explicit_step (XCD-contiguous tile order): per row
-// the same loads (w, hu, BY, BA of row y+2; BX and the ten Adams-Bashforth history planes of
-// row y; hv from the band's transposed staging) and stores (w, D_x, three dU/dt and two
-// (F*, G*) planes of row y; D_y transposed after the band), of values that only keep the
-// loads live.  Layouts: 0 the stencil's (60-column strips: 64 lanes with two halo lanes per
+// the same loads (w, hu, BY, BA of row y+2; BX and the two Adams-Bashforth history slots of
+// row y, each a quad plane and a G* plane; hv from the band's transposed staging) and stores
+// (w, D_x and the new history slot of row y; D_y transposed after the band), of values that
+// only keep the loads live.  Layouts: 0 the stencil's (60-column strips: 64 lanes with two halo lanes per
 // side); 1 the same with hv read and D_y written row-major (no transposed accesses); 2
 // 64-column strips aligned to 128-byte lines, every lane storing.  Four waves per SIMD, as
 // the stencil.
 struct PatternPlanes {
     const float *w, *hu, *BY, *BA, *BX, *hvT, *hvR;
-    const float *hist[10];
-    float *out[7];
+    const float4 *hq[2];     // history slots of steps n-1, n-2: quad planes
+    const float *h1[2];      // and their G* planes
+    float *wn, *Dx;
+    float4 *oq;              // new history slot
+    float *o1;
     float *DyT, *DyR, *sink;
     int nx, ny, P, PT, nstrips, nbands, stride;
 };
@@ -107,24 +110,30 @@ __global__ __launch_bounds__(64, 4) void probe_pattern(PatternPlanes p)
     float acc = 0.f;
     for (int j = 0; j < 4; ++j) acc += ld(p.w, ys - 2 + j) + ld(p.BA, ys - 2 + j) + ld(p.hu, ys - 2 + j) + ld(p.BY, ys - 2 + j);
     float nw = ld(p.w, ys + 2), nhu = ld(p.hu, ys + 2), nBY = ld(p.BY, ys + 2), nBA = ld(p.BA, ys + 2), nBX = ld(p.BX, ys);
-    float nh[10];
+    float4 nq[2];
+    float n1[2];
+    auto ldh = [&](int row) {
 #pragma unroll
-    for (int k = 0; k < 10; ++k) nh[k] = ld(p.hist[k], ys);
+        for (int k = 0; k < 2; ++k) {
+            nq[k] = p.hq[k][(size_t)row * P + xc];
+            n1[k] = ld(p.h1[k], row);
+        }
+    };
+    ldh(ys);
     for (int y = ys; y < ye; ++y) {
         const float v = nw + nhu + nBY + nBA + nBX + hv[y + 2 - (ys - 2)][lane];
         float hs = 0.f;
 #pragma unroll
-        for (int k = 0; k < 10; ++k) hs += nh[k];
+        for (int k = 0; k < 2; ++k) hs += (nq[k].x + nq[k].y) + (nq[k].z + nq[k].w) + n1[k];
         const int yn = min(y + 3, p.ny + 3), yc = min(y + 1, ye - 1);
         nw = ld(p.w, yn); nhu = ld(p.hu, yn); nBY = ld(p.BY, yn); nBA = ld(p.BA, yn); nBX = ld(p.BX, yc);
-#pragma unroll
-        for (int k = 0; k < 10; ++k) nh[k] = ld(p.hist[k], yc);
+        ldh(yc);
         if (out) {
             const size_t i = (size_t)y * P + x;
-            p.out[0][i] = v;
-            p.out[1][LAYOUT == 2 ? i : i - 2] = hs;
-#pragma unroll
-            for (int k = 2; k < 7; ++k) p.out[k][i] = v + (float)k * hs;
+            p.wn[i] = v;
+            p.Dx[LAYOUT == 2 ? i : i - 2] = hs;
+            p.oq[i] = make_float4(v + 2.f * hs, v + 3.f * hs, v + 4.f * hs, v + 5.f * hs);
+            p.o1[i] = v + 6.f * hs;
             if (LAYOUT == 1) p.DyR[i] = v * hs;
             else dy[y - ys][lane] = v * hs;
         }
@@ -253,7 +262,8 @@ void check(hipError_t e, const char *what)
 
 // The stencil's access pattern (probe_pattern, layout 0..2) on an nx x ny grid with R-row
 // bands (8, 12 or 16): `launches` timed launches after one warm-up; returns the average
-// launch time in ms.  Allocates its own planes (26 of (nx + 4) rows, about 450 MB at 2048^2).
+// launch time in ms.  Allocates its own planes (the equivalent of 26 of (nx + 4) rows,
+// about 450 MB at 2048^2).
 float probe_stencil_pattern(int nx, int ny, int rows, int layout, int launches)
 {
     if (nx < 4 || ny < 4 || nx > 8192 || ny > 8192) throw Error(CEL_EINVAL, "grid must lie in [4, 8192] per side");
@@ -292,8 +302,12 @@ float probe_stencil_pattern(int nx, int ny, int rows, int layout, int launches)
         check(hipEventCreate(&e1), "hipEventCreate");
         q.w = alloc(plane); q.hu = alloc(plane); q.BY = alloc(plane); q.BA = alloc(plane); q.BX = alloc(plane);
         q.hvT = alloc(tplane); q.hvR = alloc(plane);
-        for (auto &h : q.hist) h = alloc(plane);
-        for (auto &o : q.out) o = alloc(plane);
+        for (int k = 0; k < 2; ++k) {
+            q.hq[k] = reinterpret_cast<const float4 *>(alloc(4 * plane));
+            q.h1[k] = alloc(plane);
+        }
+        q.wn = alloc(plane); q.Dx = alloc(plane);
+        q.oq = reinterpret_cast<float4 *>(alloc(4 * plane)); q.o1 = alloc(plane);
         q.DyT = alloc(tplane); q.DyR = alloc(plane); q.sink = alloc(64 * sizeof(float));
         const dim3 grid(8 * ((q.nstrips * q.nbands + 7) / 8)), block(64);
         auto launch = [&]() {
