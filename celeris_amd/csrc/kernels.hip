@@ -650,14 +650,18 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
         yf_s0 = yf0; yf_s1 = yf1; yf_s2 = yf2;
     };
     // fast variant: two rows per trip, so that the window shifts between them become register
-    // renames (the plain variant's longer arithmetic would need more than 128 VGPRs for it)
+    // renames (the plain variant's longer arithmetic would need more than 128 VGPRs for it).
+    // An odd row runs first, so no value of the prologue stays live past the paired loop.
     int y = ys;
-    if (FAST)
-        for (; y + 1 < ye; y += 2) {
+    if (FAST) {
+        if ((ye - ys) & 1) row(y++);
+        for (; y < ye; y += 2) {
             row(y);
             row(y + 1);
         }
-    for (; y < ye; ++y) row(y);
+    } else {
+        for (; y < ye; ++y) row(y);
+    }
     // one atomic per wave that saw a non-finite value
     if (__ballot(bad) != 0ull && lane == 0 && p.status) atomicOr(p.status, 1);
     // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
