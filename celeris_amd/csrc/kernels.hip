@@ -854,14 +854,14 @@ __device__ __forceinline__ void load_bed(const real *src, int base, int n, real 
 
 // Level-0 equations of a chunk: A, B, C from the depths at offsets -2..+2 (line_kq,
 // line_abc: the same operations as the explicit kernel), right-hand sides from dv.
-template <int C, bool FAST>
+template <int C, bool FAST, bool FULL>
 __device__ __forceinline__ void chunk_equations(const LineBed &lb, const real *dd, const real *dv, int base, int n,
                                                 Eq *q)
 {
 #pragma unroll
     for (int i = 0; i < C; ++i) {
         const int el = base + i;
-        if (el < n) {
+        if (FULL || el < n) {
             real K1, Q;
             line_kq<FAST>(lb.kcoef, lb.d2, lb.rd2, dd[i], dd[i + 1], dd[i + 2], dd[i + 3], dd[i + 4], K1, Q);
             line_abc(K1, Q, el == 0, el == n - 1, q[i].a, q[i].b, q[i].c);
@@ -878,7 +878,9 @@ __device__ __forceinline__ void chunk_equations(const LineBed &lb, const real *d
 // ghost-inclusive output line + 2 (8-byte aligned).
 // comp: the solution is an increment; out/out_lo receive the two-real sum of the previous
 // state (prev, prev_lo, same layout as out) and the increment.
-template <int C, int W>
+// FULL: n == 64 W C, so every position of the block lies on the line and the tests against
+// the line's end drop out (the same eliminations).
+template <int C, int W, bool FULL>
 __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBed &lb, real *out,
                                               LineSmem<C, W> &sm, bool comp, const real *prev, const real *prev_lo,
                                               real *out_lo)
@@ -892,12 +894,12 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
     Eq q[C];
     {
         real dv[C], dd[C + 4];
-        load_chunk<C>(D, base, n, dv);
-        load_bed<C>(lb.B, base, n, dd);
+        load_chunk<C>(D, base, FULL ? base + C : n, dv);
+        load_bed<C>(lb.B, base, FULL ? base + C : n, dd);
 #pragma unroll
         for (int i = 0; i < C + 4; ++i) dd[i] = fmaxr(0.f, lb.cs - dd[i]);
-        if (lb.fast) chunk_equations<C, true>(lb, dd, dv, base, n, q);
-        else chunk_equations<C, false>(lb, dd, dv, base, n, q);
+        if (lb.fast) chunk_equations<C, true, FULL>(lb, dd, dv, base, n, q);
+        else chunk_equations<C, false, FULL>(lb, dd, dv, base, n, q);
     }
     const Eq margin{0.f, 1.f, 0.f, 0.f};
     // ---- reduction inside the chunk: levels 1..K0
@@ -909,13 +911,13 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
         __syncthreads();
         Eq nr = sm.xch[k & 1][t + 1 < NT ? t + 1 : t];   // one 16-byte read, then the margin
         if (t + 1 >= NT) nr = margin;
-        if (base + C - 1 + s >= n) nr = margin;
+        if (!FULL && base + C - 1 + s >= n) nr = margin;
 #pragma unroll
         for (int j = 0; j < (C >> k); ++j) {
             const int pp = ((j + 1) << k) - 1;
-            if (base + pp >= n) continue;
+            if (!FULL && base + pp >= n) continue;
             const Eq &L = q[pp - s];
-            const Eq R = (pp + s < C) ? ((base + pp + s < n) ? q[pp + s] : margin) : nr;
+            const Eq R = (pp + s < C) ? ((FULL || base + pp + s < n) ? q[pp + s] : margin) : nr;
             q[pp] = reduce_eq(L, q[pp], R);
         }
     }
@@ -928,9 +930,9 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
     for (int j = 1; j <= KT; ++j) {
         const int s = 1 << (j - 1);
         const int i = ((t + 1) << j) - 1;
-        if (i < NT && i * C + C - 1 < n) {
+        if (i < NT && (FULL || i * C + C - 1 < n)) {
             const Eq L = sm.top(i - s);
-            const Eq R = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.top(i + s) : margin;
+            const Eq R = (i + s < NT && (FULL || (i + s) * C + C - 1 < n)) ? sm.top(i + s) : margin;
             sm.set_top(i, reduce_eq(L, sm.top(i), R));
         }
         __syncthreads();
@@ -941,9 +943,9 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
     for (int j = KT; j >= 0; --j) {
         const int s = 1 << j;
         const int i = ((2 * t + 1) << j) - 1;
-        if (i < NT && i * C + C - 1 < n) {
+        if (i < NT && (FULL || i * C + C - 1 < n)) {
             const real xl = i - s >= 0 ? sm.x(i - s) : 0.f;
-            const real xr = (i + s < NT && (i + s) * C + C - 1 < n) ? sm.x(i + s) : 0.f;
+            const real xr = (i + s < NT && (FULL || (i + s) * C + C - 1 < n)) ? sm.x(i + s) : 0.f;
             const Eq e = sm.top(i);
             sm.set_x(i, (e.d - e.a * xl - e.c * xr) / e.b);
         }
@@ -958,13 +960,13 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
 #pragma unroll
         for (int qq = 0; qq < (C >> (k + 1)); ++qq) {
             const int pp = s * (2 * qq + 1) - 1;
-            if (base + pp >= n) continue;
+            if (!FULL && base + pp >= n) continue;
             const real xl = (pp - s >= 0) ? q[pp - s].d : prev_last;
-            const real xr = (base + pp + s < n) ? q[pp + s].d : 0.f;
+            const real xr = (FULL || base + pp + s < n) ? q[pp + s].d : 0.f;
             q[pp].d = (q[pp].d - q[pp].a * xl - q[pp].c * xr) / q[pp].b;
         }
     }
-    if (comp && C % 2 == 0 && base + C <= n) {
+    if (comp && C % 2 == 0 && (FULL || base + C <= n)) {
 #pragma unroll
         for (int i = 0; i < C; i += 2) {
             const Pair pv = *reinterpret_cast<const Pair *>(prev + base + i);
@@ -984,7 +986,7 @@ __device__ __forceinline__ void cr_line_block(int n, const real *D, const LineBe
                 out[base + i] = hi;
                 out_lo[base + i] = lo;
             }
-    } else if (C % 2 == 0 && base + C <= n) {
+    } else if (C % 2 == 0 && (FULL || base + C <= n)) {
 #pragma unroll
         for (int i = 0; i < C; i += 2) *reinterpret_cast<Pair *>(out + base + i) = Pair{q[i].d, q[i + 1].d};
     } else {
@@ -1036,9 +1038,11 @@ __global__ __launch_bounds__(64 * W) void solve_lines_block(const LineParams p, 
     const real *D = xdir ? p.Dx : p.DyT;
     real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
     const LineBed lb = line_bed(p, xdir, line);
-    cr_line_block<C, W>(n, D + o, lb, out, sm, COMP, COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
-                        COMP ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
-                        COMP ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
+    const real *prev = COMP ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr;
+    const real *prev_lo = COMP ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr;
+    real *out_lo = COMP ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr;
+    if (n == 64 * W * C) cr_line_block<C, W, true>(n, D + o, lb, out, sm, COMP, prev, prev_lo, out_lo);
+    else cr_line_block<C, W, false>(n, D + o, lb, out, sm, COMP, prev, prev_lo, out_lo);
 }
 
 // ------------------------------------------------------------------ boundaries
@@ -1360,9 +1364,9 @@ __global__ __launch_bounds__(64 * W) void solve_batched(const SolveJob *__restri
     real *out = (xdir ? p.hu_out : p.hvT_out) + o + 2;
     const bool comp = J.comp != 0;
     const LineBed lb = line_bed(p, xdir, line);
-    cr_line_block<C, W>(n, D + o, lb, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
-                        comp ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
-                        comp ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
+    cr_line_block<C, W, false>(n, D + o, lb, out, sm, comp, comp ? (xdir ? p.hu_prev : p.hvT_prev) + o + 2 : nullptr,
+                               comp ? (xdir ? p.hu_prev_lo : p.hvT_prev_lo) + o + 2 : nullptr,
+                               comp ? (xdir ? p.hu_lo_out : p.hvT_lo_out) + o + 2 : nullptr);
 }
 
 __global__ __launch_bounds__(256) void boundary_batched(const BoundaryJob *__restrict__ jobs,

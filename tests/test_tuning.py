@@ -124,3 +124,21 @@ def test_fast_arith_ruled_out_by_mesh_and_settings():
     e = ShallowWaterEngine(s, [Boundary(SOLID, 2)] * 4, bottom, S.initial_state(s, bottom, []))
     assert e.tuning()["fast_arith"] == 0
     e.close()
+
+
+@pytest.mark.parametrize("nx,ny,tuning", [(256, 128, (8, 4, 1, 2, 1)), (128, 512, (12, 1, 2, 8, 1)),
+                                          (512, 64, (8, 8, 1, 1, 1))])
+def test_lines_filling_their_blocks_match_oracle(nx, ny, tuning):
+    # lines of exactly 64 W C unknowns take the solver's full-line form (no tests against the
+    # line's end); the same eliminations, so the FP32 oracle's bits
+    from oracle.binding import Oracle
+    sc = S.small(nx=nx, ny=ny, seed=47, soliton=True,
+                 bnd=[Boundary(SPONGE, 9), Boundary(SOLID, 2), Boundary(SOLID, 2), Boundary(SPONGE, 7)])
+    e = ShallowWaterEngine(sc.s, sc.boundaries, sc.bottom, sc.state)
+    e.set_tuning(*tuning)
+    e.timestep(30)
+    a = e.get_state()
+    e.close()
+    o = Oracle(sc.s, sc.boundaries, sc.bottom, sc.state, "f32")
+    o.step(30)
+    assert np.array_equal(a, o.state())
