@@ -60,8 +60,10 @@ def summarize(records, makespan):
 def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=False, graph_steps=0,
           policy="fair", gpu_cells=2048 * 2048, clock=time.perf_counter, sleep=time.sleep):
     """Runs the request stream in real time through one Batch.  Each iteration launches a
-    block of at most `block` steps (ending when the first running request completes), sets
-    up at most one waiting request while it runs, then waits for it.  graph_steps > 0 replays
+    block that ends when the first running request completes, at most `block` steps long
+    while requests are still to arrive or wait for admission (so that they join soon), sets
+    up at most one waiting request while it runs, then waits for it.  Every block ends in a
+    synchronisation and a new work-list, so blocks are no shorter than they need to be.  graph_steps > 0 replays
     captured blocks of that many steps (the setup then waits for the block).  policy "fair"
     or "srpt" (see the module notes).  Returns (records, summary, states) where states maps
     request id to its final state (FP64 view of the state, if keep_states)."""
@@ -133,7 +135,9 @@ def serve(requests, hbm_budget=HBM_BYTES, block=50, max_active=512, keep_states=
                     cells += active[sid][3].cells
                 for sid in active:
                     b.set_paused(sid, sid not in run)
-            k = min(block, min(active[sid][1] for sid in run))
+            k = min(active[sid][1] for sid in run)
+            if pending or queue:
+                k = min(k, block)
             b.step(k)                     # asynchronous: the block runs while the host sets up
             if overlap:
                 arrivals()
