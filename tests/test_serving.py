@@ -222,3 +222,22 @@ def test_serve_logic_budget_srpt_and_eviction_on_cpu(monkeypatch):
     assert summ["failed"] == 1 and summ["requests"] == 3
     failed = [r for r in recs if r.failed]
     assert failed[0].rid == 99 and failed[0].finished < max(r.finished for r in recs if not r.failed)
+
+
+def test_blocks_run_to_the_next_completion_once_nothing_waits(monkeypatch):
+    """While requests wait for admission, blocks are at most `block` steps long (one
+    admission per block); once every request is resident, each block ends exactly at the next
+    completion."""
+    reqs = [S.Request(i, 0.0, S.small(nx=16 + 4 * i, ny=16, seed=i), 40 + 30 * i) for i in range(4)]
+    (recs, summ, _), b = _fake_serve(monkeypatch, reqs, block=8)
+    assert summ["requests"] == 4 and summ["failed"] == 0
+    ks = [k for k, _ in b.log]
+    admitted_all = next(j for j, (_, run) in enumerate(b.log) if len(run) == 4)
+    assert all(k <= 8 for k in ks[:admitted_all])
+    # after the last admission: one block per completion, each running to it
+    rest = b.log[admitted_all:]
+    assert len(rest) == 4
+    assert [len(run) for _, run in rest] == [4, 3, 2, 1]
+    # every request advanced exactly its steps (members were admitted in arrival order)
+    for sid, r in enumerate(reqs, start=1):
+        assert sum(k for k, run in b.log if sid in run) == r.steps
