@@ -412,7 +412,7 @@ def main():
     ap.add_argument("--autotune", type=int, default=30,
                     help="repeats per candidate when autotuning band height and line-solve shapes (0: defaults); "
                          "candidates within a few per cent of each other need about 30 to be told apart")
-    ap.add_argument("--band-rows", type=int, default=0, choices=[0, 4, 8, 12, 16],
+    ap.add_argument("--band-rows", type=int, default=0, choices=[0, 2, 4, 8, 12, 16],
                     help="pin the explicit kernel's band height after autotuning (0: keep the autotuned one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=10000,

@@ -112,6 +112,13 @@ struct alignas(16) Vec { real v[kVec]; };
 struct alignas(2 * sizeof(real)) Pair { real x, y; };
 // the four history values of one cell in a quad plane (one 16-byte access in FP32)
 struct alignas(4 * sizeof(real)) Quad { real v[4]; };
+// the pieces a band of R rows stages and stores its transposed columns in: 16 bytes, or
+// two elements for 2-row bands (whose first row is only even)
+template <int R>
+struct alignas(sizeof(real) * (R % kVec == 0 ? kVec : 2)) BandPiece {
+    static constexpr int n = R % kVec == 0 ? kVec : 2;
+    real v[n];
+};
 
 // ------------------------------------------------------------------ helpers
 
@@ -348,23 +355,26 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     const int ye = min(ys + R, p.ny + 2);
     const int P = p.P, PT = p.PT;
 
-    // transposed hv: each lane stages rows [ys-2, ye+2) of its own column (16-byte loads;
-    // ys-2 is a multiple of 4 and the column pitch leaves >= 16 elements of padding)
+    // transposed hv: each lane stages rows [ys-2, ye+2) of its own column in pieces of E
+    // elements (16 bytes; 8 bytes for 2-row bands, whose ys-2 is only even); ys-2 is a
+    // multiple of E and the column pitch leaves >= 16 elements of padding
+    using Piece = BandPiece<R>;
+    constexpr int E = Piece::n;
     {
         const int nrows = ye - ys + 4;
-        const Vec *src = reinterpret_cast<const Vec *>(p.hvT + (size_t)xc * PT + (ys - 2));
-        for (int r = 0; r < nrows; r += kVec) {
-            const Vec v = src[r / kVec];
+        const Piece *src = reinterpret_cast<const Piece *>(p.hvT + (size_t)xc * PT + (ys - 2));
+        for (int r = 0; r < nrows; r += E) {
+            const Piece v = src[r / E];
 #pragma unroll
-            for (int j = 0; j < kVec; ++j)
+            for (int j = 0; j < E; ++j)
                 if (r + j < R + 4) hvblk[r + j][lane] = v.v[j];
         }
         if (SCHEME == 2) {
-            const Vec *csrc = reinterpret_cast<const Vec *>(p.ChvT + (size_t)xc * PT + (ys - 2));
-            for (int r = 0; r < nrows; r += kVec) {
-                const Vec v = csrc[r / kVec];
+            const Piece *csrc = reinterpret_cast<const Piece *>(p.ChvT + (size_t)xc * PT + (ys - 2));
+            for (int r = 0; r < nrows; r += E) {
+                const Piece v = csrc[r / E];
 #pragma unroll
-                for (int j = 0; j < kVec; ++j)
+                for (int j = 0; j < E; ++j)
                     if (r + j < R + 4) chvblk[r + j][lane] = v.v[j];
             }
         }
@@ -664,17 +674,17 @@ __device__ __forceinline__ void explicit_tile(const ExplicitParams &p, int tile,
     }
     // one atomic per wave that saw a non-finite value
     if (__ballot(bad) != 0ull && lane == 0 && p.status) atomicOr(p.status, 1);
-    // column x, rows [ys, ye) at offset y-2: each lane writes its column in 16-byte pieces
+    // column x, rows [ys, ye) at offset y-2: each lane writes its column in pieces of E
     if (out_lane) {
         real *dst = p.DyT + (size_t)x * PT + (ys - 2);
         const int nr = ye - ys;
         if (nr == R) {
-            Vec *dv = reinterpret_cast<Vec *>(dst);
-            for (int r = 0; r < R; r += kVec) {
-                Vec v;
+            Piece *dv = reinterpret_cast<Piece *>(dst);
+            for (int r = 0; r < R; r += E) {
+                Piece v;
 #pragma unroll
-                for (int j = 0; j < kVec; ++j) v.v[j] = dystg[r + j][lane];
-                dv[r / kVec] = v;
+                for (int j = 0; j < E; ++j) v.v[j] = dystg[r + j][lane];
+                dv[r / E] = v;
             }
         } else {
             for (int r = 0; r < nr; ++r) dst[r] = dystg[r][lane];
@@ -1736,11 +1746,12 @@ hipError_t launch_explicit(int scheme, bool comp, int rows, bool fast, const Exp
 #define CEL_EXPLICIT_R(S, CP)                                  \
     do {                                                       \
         if (rows == 4) CEL_EXPLICIT(S, CP, 4);                 \
+        else if (rows == 2) CEL_EXPLICIT(S, CP, 2);            \
         else if (rows == 12) CEL_EXPLICIT(S, CP, 12);          \
         else if (rows == 16) CEL_EXPLICIT(S, CP, 16);          \
         else CEL_EXPLICIT(S, CP, 8);                           \
     } while (0)
-    if (rows != 4 && rows != 8 && rows != 12 && rows != 16) return hipErrorInvalidValue;
+    if (rows != 2 && rows != 4 && rows != 8 && rows != 12 && rows != 16) return hipErrorInvalidValue;
     if (comp) {
         switch (scheme) {
         case 0: CEL_EXPLICIT_R(0, true); break;

@@ -364,7 +364,7 @@ int cel_batch_time(cel_batch *b, int id, double *t);
  * every elimination of the line solves is computed the same way whatever the choice.  No
  * counterpart in the reference. */
 typedef struct {
-    int band_rows;              /* rows per wave in the explicit stencil: 4, 8, 12 or 16 */
+    int band_rows;              /* rows per wave in the explicit stencil: 2, 4, 8, 12 or 16 */
     int x_chunk, x_waves;       /* x-line solve: unknowns per thread (1..16), waves per line (1..8) */
     int y_chunk, y_waves;       /* y-line solve */
     float explicit_us, x_lines_us, y_lines_us;   /* timings behind an autotuned choice (else 0) */

@@ -31,14 +31,15 @@ def run(sc, steps, tuning=None, autotune=False):
     e.timestep(steps // 2)
     if autotune:
         t = e.autotune(repeats=3)
-        assert t["band_rows"] in (4, 8, 12, 16) and t["explicit_us"] > 0
+        assert t["band_rows"] in (2, 4, 8, 12, 16) and t["explicit_us"] > 0
     e.timestep(steps - steps // 2)
     st = e.get_state_f64()
     e.close()
     return st
 
 
-@pytest.mark.parametrize("tuning", [(4, 1, 2, 2, 2), (16, 2, 1, 1, 2), (8, 1, 2, 4, 1), (16, 4, 4, 8, 2), (12, 2, 1, 2, 1)])
+@pytest.mark.parametrize("tuning", [(4, 1, 2, 2, 2), (16, 2, 1, 1, 2), (8, 1, 2, 4, 1), (16, 4, 4, 8, 2), (12, 2, 1, 2, 1),
+                                    (2, 2, 1, 2, 1), (2, 2, 1, 2, 1, False)])
 def test_fixed_tunings_bit_identical(tuning):
     sc = case(100, 70, 41)
     assert np.array_equal(run(sc, 60), run(sc, 60, tuning=tuning))
@@ -127,7 +128,7 @@ def test_fast_arith_ruled_out_by_mesh_and_settings():
 
 
 @pytest.mark.parametrize("nx,ny,tuning", [(256, 128, (8, 4, 1, 2, 1)), (128, 512, (12, 1, 2, 8, 1)),
-                                          (512, 64, (8, 8, 1, 1, 1))])
+                                          (512, 64, (8, 8, 1, 1, 1)), (256, 129, (2, 4, 1, 4, 1))])
 def test_lines_filling_their_blocks_match_oracle(nx, ny, tuning):
     # lines of exactly 64 W C unknowns take the solver's full-line form (no tests against the
     # line's end); the same eliminations, so the FP32 oracle's bits
