@@ -31,8 +31,9 @@ using real = double;
 using real = float;
 #endif
 
-// Rows one wave marches through in the explicit kernel: 4, 8 or 16 (chosen per engine by
-// the autotuner; 8 by default).
+// Rows one wave marches through in the explicit kernel: 2, 4, 8, 12 or 16 for an engine (by
+// default the tallest of 12 and 8 whose bands fill the device, else 2), 4, 8 or 16 for a
+// batch (8 until its autotuner picks).
 constexpr int kDefaultBandRows = 8;
 constexpr int kStripCols = 60;  // interior columns one wave produces (64 lanes - 2 halo each side)
 
